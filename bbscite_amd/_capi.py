@@ -1,0 +1,387 @@
+"""ctypes binding of the C-ABI in include/bbscite.h (libbbscite.so).
+
+The shared library is built in-tree by bbscite_amd.build (hipcc for gfx950)
+and loaded from this package directory. There is no fallback: if the
+library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbbscite.so")
+
+MAX_GENES = 127
+MAX_CELLS = 128
+MAX_NODES = 128
+
+OK = 0
+ERR_ARG = -1
+ERR_SIZE = -2
+ERR_HIP = -3
+ERR_BUDGET = -4
+ERR_STATE = -5
+ERR_NODEV = -6
+
+MOVE_CHANGE_BETA = 0
+MOVE_PRUNE_REATTACH = 1
+MOVE_SWAP_NODES = 2
+MOVE_SWAP_SUBTREES = 3
+
+SCORE_NO_PRIOR = 1
+
+PURPOSE_INIT_TREE = 2
+PURPOSE_TREE_SAMPLE = 6
+
+
+class Params(C.Structure):
+    """bbs_params (include/bbscite.h); defaults of src/Parameters.hpp:39-45."""
+
+    _fields_ = [
+        ("alpha_mean", C.c_float),
+        ("beta_mean", C.c_float),
+        ("beta_sd", C.c_float),
+        ("gamma", C.c_float),
+        ("prob_beta_change", C.c_float),
+        ("prob_prune_n_reattach", C.c_float),
+        ("prob_swap_nodes", C.c_float),
+        ("prob_swap_subtrees", C.c_float),
+        ("beta_jump_scaling_chi", C.c_float),
+        ("n_chains", C.c_uint32),
+        ("n_steps", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class StepTrace(C.Structure):
+    _fields_ = [
+        ("move_type", C.c_uint32),
+        ("v", C.c_uint32),
+        ("w", C.c_uint32),
+        ("parent_of_v", C.c_uint32),
+        ("parent_of_w", C.c_uint32),
+        ("descendant_of_v", C.c_uint32),
+        ("nondescendant_of_v", C.c_uint32),
+        ("new_beta", C.c_float),
+        ("proposed_score", C.c_float),
+        ("flags", C.c_uint32),
+    ]
+
+
+TRACE_DTYPE = np.dtype(
+    [
+        ("move_type", "<u4"),
+        ("v", "<u4"),
+        ("w", "<u4"),
+        ("parent_of_v", "<u4"),
+        ("parent_of_w", "<u4"),
+        ("descendant_of_v", "<u4"),
+        ("nondescendant_of_v", "<u4"),
+        ("new_beta", "<f4"),
+        ("proposed_score", "<f4"),
+        ("flags", "<u4"),
+    ]
+)
+
+_P = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+
+# name: (restype, argtypes)
+_SIGS = {
+    "bbs_params_default": (None, [C.POINTER(Params)]),
+    "bbs_create": (C.c_int, [C.POINTER(Params), _u32, _u32, _P, _P, C.POINTER(_P)]),
+    "bbs_set_initial_states": (C.c_int, [_P, _P, _P, _P, _P]),
+    "bbs_set_trace": (C.c_int, [_P, _u32]),
+    "bbs_run": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "bbs_begin": (C.c_int, [_P]),
+    "bbs_advance": (C.c_int, [_P, _u32, C.POINTER(C.c_float)]),
+    "bbs_get_best": (C.c_int, [_P, _P, _P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "bbs_get_chain_results": (C.c_int, [_P, _P, _P, _P, _P]),
+    "bbs_get_chain_states": (C.c_int, [_P, _P, _P]),
+    "bbs_get_trace": (C.c_int, [_P, _P]),
+    "bbs_score_trees": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
+    "bbs_bench_score_random": (C.c_int, [_P, _u64, _u32, _u32, _u32, _P, _P, _u32]),
+    "bbs_random_trees_device": (C.c_int, [_P, _u64, _u32, _u32, _P, _P]),
+    "bbs_eval_libm": (C.c_int, [_P, _P, _u32, _P, _P, _P]),
+    "bbs_last_error": (C.c_char_p, [_P]),
+    "bbs_destroy": (None, [_P]),
+    "bbs_parse_csv": (C.c_int, [C.c_char_p, _u64, _P, C.POINTER(_u32), C.POINTER(_u32)]),
+    "bbs_encode_matrix": (C.c_int, [_P, _u32, _u32, _P, _P]),
+    "bbs_generate_dataset": (
+        C.c_int,
+        [_u64, _u32, _u32, C.c_float, C.c_float, C.c_float, _P, _P, _P],
+    ),
+    "bbs_random_trees": (C.c_int, [_u64, _u32, _u32, _P]),
+    "bbs_initial_tree": (C.c_int, [_u64, _u32, _u32, _u32, _P, _P]),
+    "bbs_parent_vector_to_matrix": (C.c_int, [_P, _u32, _P, _P]),
+    "bbs_pruefer_to_parent_vector": (C.c_int, [_P, _u32, _P]),
+    "bbs_tree_to_graphviz": (C.c_int64, [_P, _P, _u32, C.c_char_p, _u64]),
+    "bbs_tree_to_newick": (C.c_int64, [_P, _P, _u32, C.c_char_p, _u64]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+class BbsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bbscite error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libbbscite.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        lib_ = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib_, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib_
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc, ctx=None):
+    if rc != OK:
+        msg = lib().bbs_last_error(ctx)
+        raise BbsError(rc, msg.decode() if msg else "")
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().bbs_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+# ---------------------------------------------------------------- host helpers
+
+
+def parse_csv(text: str | bytes):
+    """Parse the reference CSV format (src/main.cpp:52-144)."""
+    if isinstance(text, str):
+        text = text.encode()
+    m = np.zeros(MAX_GENES * MAX_CELLS, np.int8)
+    g, c = _u32(), _u32()
+    _check(lib().bbs_parse_csv(text, len(text), _ptr(m), C.byref(g), C.byref(c)))
+    return m[: g.value * c.value].reshape(g.value, c.value).copy()
+
+
+def encode_matrix(matrix: np.ndarray):
+    """genes x cells int8 -> (is_mutated, is_known) as [cells][2] uint64."""
+    matrix = np.ascontiguousarray(matrix, np.int8)
+    G, Cn = matrix.shape
+    mut = np.zeros((Cn, 2), np.uint64)
+    kno = np.zeros((Cn, 2), np.uint64)
+    _check(lib().bbs_encode_matrix(_ptr(matrix), G, Cn, _ptr(mut), _ptr(kno)))
+    return mut, kno
+
+
+def generate_dataset(seed, n_genes, n_cells, fd, ad, missing):
+    """Synthetic data: returns (matrix[G][C] int8, true parent[N], attachment[C])."""
+    m = np.zeros((n_genes, n_cells), np.int8)
+    par = np.zeros(n_genes + 1, np.uint32)
+    att = np.zeros(n_cells, np.uint32)
+    _check(
+        lib().bbs_generate_dataset(
+            seed, n_genes, n_cells, fd, ad, missing, _ptr(m), _ptr(par), _ptr(att)
+        )
+    )
+    return m, par, att
+
+
+def random_trees(seed, n_genes, n_trees):
+    dm = np.zeros((n_trees, n_genes + 1, 2), np.uint64)
+    _check(lib().bbs_random_trees(seed, n_genes, n_trees, _ptr(dm)))
+    return dm
+
+
+def initial_tree(seed, job, chain, n_genes):
+    am = np.zeros((n_genes + 1, 2), np.uint64)
+    dm = np.zeros((n_genes + 1, 2), np.uint64)
+    _check(lib().bbs_initial_tree(seed, job, chain, n_genes, _ptr(am), _ptr(dm)))
+    return am, dm
+
+
+def parent_vector_to_matrix(parent):
+    parent = np.ascontiguousarray(parent, np.uint32)
+    n = len(parent)
+    am = np.zeros((n, 2), np.uint64)
+    dm = np.zeros((n, 2), np.uint64)
+    _check(lib().bbs_parent_vector_to_matrix(_ptr(parent), n, _ptr(am), _ptr(dm)))
+    return am, dm
+
+
+def pruefer_to_parent_vector(code):
+    code = np.ascontiguousarray(code, np.uint32)
+    par = np.zeros(len(code) + 2, np.uint32)
+    _check(lib().bbs_pruefer_to_parent_vector(_ptr(code), len(code), _ptr(par)))
+    return par
+
+
+def _text(fn, am, dm):
+    am = np.ascontiguousarray(am, np.uint64)
+    dm = np.ascontiguousarray(dm, np.uint64)
+    n = dm.shape[0]
+    cap = 64 * n + 256
+    buf = C.create_string_buffer(cap)
+    ln = fn(_ptr(am), _ptr(dm), n, buf, cap)
+    if ln < 0:
+        _check(int(ln))
+    return buf.value.decode()
+
+
+def tree_to_graphviz(am, dm):
+    return _text(lib().bbs_tree_to_graphviz, am, dm)
+
+
+def tree_to_newick(am, dm):
+    return _text(lib().bbs_tree_to_newick, am, dm)
+
+
+# ---------------------------------------------------------------- device
+
+
+class Context:
+    """Owns one bbs_ctx (the Application counterpart, src/Application.hpp:88-256)."""
+
+    def __init__(self, params: Params, is_mutated, is_known, n_genes):
+        self.params = params
+        self.is_mutated = np.ascontiguousarray(is_mutated, np.uint64)
+        self.is_known = np.ascontiguousarray(is_known, np.uint64)
+        self.n_cells = self.is_mutated.shape[0]
+        self.n_genes = n_genes
+        self.n_nodes = n_genes + 1
+        h = C.c_void_p()
+        rc = lib().bbs_create(
+            C.byref(params),
+            self.n_cells,
+            n_genes,
+            _ptr(self.is_mutated),
+            _ptr(self.is_known),
+            C.byref(h),
+        )
+        _check(rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bbs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self.h)
+
+    def set_initial_states(self, am, dm, beta, score=None):
+        am = np.ascontiguousarray(am, np.uint64)
+        dm = np.ascontiguousarray(dm, np.uint64)
+        beta = np.ascontiguousarray(beta, np.float32)
+        sc = None if score is None else np.ascontiguousarray(score, np.float32)
+        self._c(lib().bbs_set_initial_states(self.h, _ptr(am), _ptr(dm), _ptr(beta), _ptr(sc)))
+
+    def set_trace(self, n):
+        self._c(lib().bbs_set_trace(self.h, n))
+
+    def run(self):
+        ms = C.c_float()
+        self._c(lib().bbs_run(self.h, C.byref(ms)))
+        return ms.value
+
+    def begin(self):
+        self._c(lib().bbs_begin(self.h))
+
+    def advance(self, n):
+        ms = C.c_float()
+        self._c(lib().bbs_advance(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def best(self):
+        am = np.zeros((self.n_nodes, 2), np.uint64)
+        dm = np.zeros((self.n_nodes, 2), np.uint64)
+        b, s = C.c_float(), C.c_float()
+        self._c(lib().bbs_get_best(self.h, _ptr(am), _ptr(dm), C.byref(b), C.byref(s)))
+        return am, dm, b.value, s.value
+
+    def chain_results(self):
+        R = self.params.n_chains
+        bs = np.zeros(R, np.float32)
+        st = np.zeros(R, np.uint32)
+        fs = np.zeros(R, np.float32)
+        fb = np.zeros(R, np.float32)
+        self._c(lib().bbs_get_chain_results(self.h, _ptr(bs), _ptr(st), _ptr(fs), _ptr(fb)))
+        return bs, st, fs, fb
+
+    def chain_states(self):
+        R = self.params.n_chains
+        am = np.zeros((R, self.n_nodes, 2), np.uint64)
+        dm = np.zeros((R, self.n_nodes, 2), np.uint64)
+        self._c(lib().bbs_get_chain_states(self.h, _ptr(am), _ptr(dm)))
+        return am, dm
+
+    def trace(self, n_traced):
+        out = np.zeros((n_traced, self.params.n_steps), TRACE_DTYPE)
+        self._c(lib().bbs_get_trace(self.h, _ptr(out)))
+        return out
+
+    def score_trees(self, dm_rows, betas=None, with_prior=True, counts=False):
+        dm_rows = np.ascontiguousarray(dm_rows, np.uint64)
+        T = dm_rows.shape[0]
+        ll = np.zeros(T, np.float32)
+        b = None if betas is None else np.ascontiguousarray(betas, np.float32)
+        cnt = np.zeros((T, self.n_cells, 4), np.uint8) if counts else None
+        flags = 0 if with_prior else SCORE_NO_PRIOR
+        self._c(lib().bbs_score_trees(self.h, _ptr(dm_rows), T, _ptr(b), flags, _ptr(ll), _ptr(cnt)))
+        return (ll, cnt) if counts else ll
+
+    def bench_score_random(self, seed, n_trees, reps, with_prior=False, n_head=0):
+        ms = np.zeros(reps, np.float32)
+        head = np.zeros(n_head, np.float32)
+        flags = 0 if with_prior else SCORE_NO_PRIOR
+        self._c(
+            lib().bbs_bench_score_random(
+                self.h, seed, n_trees, reps, flags, _ptr(ms), _ptr(head) if n_head else None, n_head
+            )
+        )
+        return ms, head
+
+    def eval_libm(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        e = np.zeros_like(x)
+        l = np.zeros_like(x)
+        l1 = np.zeros_like(x)
+        self._c(lib().bbs_eval_libm(self.h, _ptr(x), len(x), _ptr(e), _ptr(l), _ptr(l1)))
+        return e, l, l1
+
+    def random_trees_device(self, seed, purpose, n_trees, with_am=False):
+        dm = np.zeros((n_trees, self.n_nodes, 2), np.uint64)
+        am = np.zeros_like(dm) if with_am else None
+        self._c(lib().bbs_random_trees_device(self.h, seed, purpose, n_trees, _ptr(am), _ptr(dm)))
+        return (am, dm) if with_am else dm

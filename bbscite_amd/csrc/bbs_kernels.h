@@ -1,0 +1,71 @@
+/*
+ * Launch interface between the C-ABI layer (bbscite_capi.cpp) and the HIP
+ * kernels (bbs_kernels.hip). Plain structs passed by value as kernel
+ * arguments; all pointers are device memory.
+ */
+#ifndef BBS_KERNELS_H
+#define BBS_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/bbscite.h"
+#include "bbs_device.h"
+
+namespace bbs {
+
+/* Arguments of one chain-kernel launch: chains [0, n_chains) of one job,
+ * steps [step_begin, step_end). Chain c uses Philox chain index
+ * chain_offset + c; states and bests are read at the start and written
+ * back at the end, so consecutive launches continue the same chains. */
+struct ChainArgs {
+  const uint64_t *cells;  /* [n_cells][4] K&M lo/hi, K&~M lo/hi      */
+  uint64_t *am, *dm;      /* [n_chains][n_nodes][2] current state     */
+  float *beta, *score;    /* [n_chains]                               */
+  uint64_t *best_am, *best_dm;
+  float *best_beta, *best_score;
+  uint32_t *best_step;    /* UINT32_MAX: no record yet                */
+  bbs_step_trace *trace;  /* [n_traced][trace_stride]                 */
+  uint32_t n_traced, trace_stride;
+  uint64_t seed;
+  uint32_t job, chain_offset;
+  uint32_t n_chains, step_begin, step_end;
+  uint32_t n_cells, n_nodes;
+  uint32_t score_init;    /* score the initial trees on the device    */
+  float L00, L10, gamma;
+  PriorArgs prior;
+  bbs_move_probs probs;
+};
+
+struct ScoreArgs {
+  const uint64_t *cells;
+  const uint64_t *dm;     /* [n_trees][n_nodes][2] */
+  const float *betas;     /* [n_trees] or null     */
+  float *ll;              /* [n_trees]             */
+  uint8_t *counts;        /* [n_trees][n_cells][4] or null */
+  uint32_t n_trees, n_cells, n_nodes, with_prior;
+  float L00, L10, L01, L11, prior_value; /* L01/L11/prior for null betas */
+  PriorArgs prior;
+};
+
+struct TreeGenArgs {
+  uint64_t *am;           /* nullable */
+  uint64_t *dm;
+  float *beta;            /* nullable */
+  float beta_value;
+  uint64_t seed;
+  uint32_t job, chain_offset, purpose;
+  uint32_t n_trees, n_nodes;
+};
+
+void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w);
+hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream);
+hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream);
+hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
+hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,
+                       hipStream_t stream);
+
+} // namespace bbs
+
+#endif /* BBS_KERNELS_H */

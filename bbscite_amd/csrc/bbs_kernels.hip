@@ -1,0 +1,586 @@
+/*
+ * HIP kernels of the chain search for gfx950.
+ *
+ *   chain_kernel   one wavefront per Markov chain; the chain's two bit
+ *                  matrices stay in VGPRs for the whole launch. One
+ *                  iteration = the WorkKernel loop body of
+ *                  src/Application.hpp:412-487: draw and realise a move,
+ *                  update the bit matrices, score the proposal, Metropolis
+ *                  step, best tracking.
+ *   score_kernel   one wavefront per tree: TreeScorer::logscore_tree
+ *                  (src/TreeScorer.hpp:131-210) over many trees.
+ *   random_trees_kernel  one wavefront per tree: uniform Pruefer code ->
+ *                  parent vector -> ancestor rows
+ *                  (src/MutationTree.hpp:379-409,427-510).
+ */
+#include <hip/hip_runtime.h>
+
+#include "bbs_device.h"
+#include "bbs_kernels.h"
+
+namespace bbs {
+
+/* ------------------------------------------------------------------ moves */
+
+/* The proposal of one move for the rows of this lane
+ * (src/MutationTree.hpp:123-307 restated on 64-bit words). */
+template <int NS>
+struct MoveCtx {
+  uint32_t type, v, w;
+  bool related;                 /* w is an ancestor of v (Dw[v])          */
+  uint64_t Dv[NS], Dw[NS];      /* descendants of v, w                    */
+  uint64_t Av[NS], Aw[NS];      /* ancestors of v, w                      */
+  uint32_t v_target, w_target;
+  uint64_t At_v[NS], At_w[NS];  /* ancestors of v_target, w_target        */
+};
+
+template <int NS>
+__device__ __forceinline__ void apply_move(const MoveCtx<NS> &m,
+                                           const uint64_t (&am)[NS][NS],
+                                           const uint64_t (&dm)[NS][NS],
+                                           uint64_t (&pam)[NS][NS],
+                                           uint64_t (&pdm)[NS][NS]) {
+  const uint32_t lane = lane_id();
+  const uint64_t lanebit = 1ull << lane;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+    const bool dvx = (m.Dv[k] & lanebit) != 0;
+    const bool dwx = (m.Dw[k] & lanebit) != 0;
+    switch (m.type) {
+    case BBS_MOVE_SWAP_NODES: {
+      /* relabel v <-> w: row x takes row sigma(x) with bits v, w exchanged */
+      const bool is_v = x == m.v, is_w = x == m.w;
+      uint64_t sa[NS], sd[NS];
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        sa[wd] = is_v ? m.Dw[wd] : (is_w ? m.Dv[wd] : am[k][wd]);
+        sd[wd] = is_v ? m.Aw[wd] : (is_w ? m.Av[wd] : dm[k][wd]);
+      }
+      const bool a_v = row_bit<NS>(sa, m.v), a_w = row_bit<NS>(sa, m.w);
+      const bool d_v = row_bit<NS>(sd, m.v), d_w = row_bit<NS>(sd, m.w);
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        const uint64_t mv = ((m.v >> 6) == (uint32_t)wd) ? 1ull << (m.v & 63u) : 0ull;
+        const uint64_t mw = ((m.w >> 6) == (uint32_t)wd) ? 1ull << (m.w & 63u) : 0ull;
+        pam[k][wd] = (sa[wd] & ~(mv | mw)) | (a_w ? mv : 0ull) | (a_v ? mw : 0ull);
+        pdm[k][wd] = (sd[wd] & ~(mv | mw)) | (d_w ? mv : 0ull) | (d_v ? mw : 0ull);
+      }
+      break;
+    }
+    case BBS_MOVE_PRUNE_REATTACH: {
+      /* x->y for y in subtree(v): x->t or (v->x and x->y)
+       * y->x for x in subtree(v): y->t or (y->x and v->y) */
+      const bool amx_t = row_bit<NS>(am[k], m.v_target);
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        const uint64_t a = am[k][wd], d = dm[k][wd], dv = m.Dv[wd];
+        pam[k][wd] = (dv & ((amx_t ? ~0ull : 0ull) | (dvx ? a : 0ull))) | (~dv & a);
+        pdm[k][wd] = dvx ? (m.At_v[wd] | (d & dv)) : d;
+      }
+      break;
+    }
+    case BBS_MOVE_SWAP_SUBTREES: {
+      const bool amx_vt = row_bit<NS>(am[k], m.v_target);
+      const bool amx_wt = row_bit<NS>(am[k], m.w_target);
+      if (m.related) {
+        /* classes: 2 = subtree(v), 1 = subtree(w) \ subtree(v), 0 = rest */
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) {
+          const uint64_t a = am[k][wd], d = dm[k][wd];
+          const uint64_t c2 = m.Dv[wd], c1 = m.Dw[wd] & ~m.Dv[wd];
+          const uint64_t c0 = ~(m.Dw[wd] | m.Dv[wd]);
+          if (dvx) {
+            pam[k][wd] = (a & c2) | (amx_wt ? c1 : 0ull);
+            pdm[k][wd] = d & ~c1;
+          } else if (dwx) {
+            pam[k][wd] = a & c1;
+            pdm[k][wd] = (d & ~c2) | (m.At_w[wd] & c2);
+          } else {
+            pam[k][wd] = a;
+            pdm[k][wd] = d & c0;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) {
+          const uint64_t a = am[k][wd], d = dm[k][wd];
+          const uint64_t dv = m.Dv[wd], dw = m.Dw[wd];
+          const uint64_t P = dv & ~dw, Q = ~dv & dw, R = ~(P | Q);
+          pam[k][wd] = (P & ((amx_vt ? ~0ull : 0ull) | (dvx ? a : 0ull))) |
+                       (Q & ((amx_wt ? ~0ull : 0ull) | (dwx ? a : 0ull))) |
+                       (R & a);
+          if (dvx && !dwx)
+            pdm[k][wd] = m.At_v[wd] | (dv & d);
+          else if (!dvx && dwx)
+            pdm[k][wd] = m.At_w[wd] | (dw & d);
+          else
+            pdm[k][wd] = d;
+        }
+      }
+      break;
+    }
+    default: /* ChangeBeta */
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        pam[k][wd] = am[k][wd];
+        pdm[k][wd] = dm[k][wd];
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------ chain kernel */
+
+template <int NS, int CS, int W, bool TRACE>
+__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
+  __shared__ __align__(16) float lds[64 * CS];
+  const uint32_t lane = lane_id();
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
+  const uint32_t N = a.n_nodes;
+
+  Cells<CS, W> cells;
+  load_cells<CS, W>(a.cells, a.n_cells, cells);
+
+  uint64_t am[NS][NS], dm[NS][NS], bam[NS][NS], bdm[NS][NS];
+  const uint64_t row_base = (uint64_t)chain * N;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) {
+      am[k][wd] = x < N ? a.am[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+      dm[k][wd] = x < N ? a.dm[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+    }
+  }
+
+  float cur_beta = a.beta[chain];
+  LogProbs L;
+  L.L00 = a.L00;
+  L.L10 = a.L10;
+  L.L01 = bbs_logf(cur_beta);
+  L.L11 = bbs_logf_of_double(1.0 - (double)cur_beta);
+  float cur_prior = logscore_beta(a.prior, cur_beta);
+  uint32_t unused01[CS], unused11[CS];
+  float cur_score;
+  if (a.score_init)
+    cur_score = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds,
+                                             unused01, unused11) +
+                cur_prior;
+  else
+    cur_score = a.score[chain];
+
+  float best_score = a.best_score[chain];
+  float best_beta = a.best_beta[chain];
+  uint32_t best_step = a.best_step[chain];
+  const bool have_best = best_step != 0xffffffffu;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) {
+      bam[k][wd] = (have_best && x < N) ? a.best_am[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+      bdm[k][wd] = (have_best && x < N) ? a.best_dm[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+    }
+  }
+
+  for (uint32_t step = a.step_begin; step < a.step_end; step++) {
+    const bbs_raw_move raw =
+        bbs_draw_move(a.seed, a.job, a.chain_offset + chain, step, N, a.probs);
+    MoveCtx<NS> m;
+    m.type = raw.move_type;
+    uint32_t v = raw.raw_v, w = raw.raw_w;
+    fetch_row<NS>(am, v, m.Dv);
+    if (row_bit<NS>(m.Dv, w)) { /* v is an ancestor of w: swap */
+      uint32_t t = v;
+      v = w;
+      w = t;
+      fetch_row<NS>(am, v, m.Dv);
+    }
+    m.v = v;
+    m.w = w;
+    fetch_row<NS>(am, w, m.Dw);
+    fetch_row<NS>(dm, v, m.Av);
+    fetch_row<NS>(dm, w, m.Aw);
+    m.related = row_bit<NS>(m.Dw, v);
+    const uint32_t parent_v = parent_of<NS>(dm, m.Av, v, N);
+    const uint32_t parent_w = parent_of<NS>(dm, m.Aw, w, N);
+    uint32_t nd_v = 0, nd_w = 0;
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) {
+      nd_v += (uint32_t)__popcll(m.Dv[wd]);
+      nd_w += (uint32_t)__popcll(m.Dw[wd]);
+    }
+    const uint32_t nn_v = N - nd_v;
+    const uint32_t i_d = (uint32_t)floorf(raw.raw_descendant_of_v * (float)nd_v);
+    const uint32_t i_n = (uint32_t)floorf(raw.raw_nondescendant_of_v * (float)nn_v);
+    const uint32_t desc_v = select_bit<NS>(m.Dv, i_d, N);
+    uint64_t ndv[NS];
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) ndv[wd] = ~m.Dv[wd];
+    const uint32_t nondesc_v = select_bit<NS>(ndv, i_n, N);
+    float nb = cur_beta + raw.beta_jump;
+    if (nb < 0.0f) nb = fabsf(nb);
+    if (nb > 1.0f) nb = nb - 2.0f * (nb - 1.0f);
+
+    m.v_target = 0u;
+    m.w_target = 0u;
+    if (m.type == BBS_MOVE_SWAP_SUBTREES) {
+      m.v_target = parent_w;
+      m.w_target = m.related ? desc_v : parent_v;
+    } else if (m.type == BBS_MOVE_PRUNE_REATTACH) {
+      m.v_target = nondesc_v;
+    }
+    fetch_row<NS>(dm, m.v_target, m.At_v);
+    fetch_row<NS>(dm, m.w_target, m.At_w);
+
+    uint64_t pam[NS][NS], pdm[NS][NS];
+    apply_move<NS>(m, am, dm, pam, pdm);
+
+    float prop_beta = cur_beta;
+    LogProbs Lp = L;
+    float prop_prior = cur_prior;
+    if (m.type == BBS_MOVE_CHANGE_BETA) {
+      prop_beta = nb;
+      Lp.L01 = bbs_logf(nb);
+      Lp.L11 = bbs_logf_of_double(1.0 - (double)nb);
+      prop_prior = logscore_beta(a.prior, nb);
+    }
+    const float prop_score =
+        score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
+                                     unused11) +
+        prop_prior;
+
+    float corr = 1.0f;
+    if (m.type == BBS_MOVE_SWAP_SUBTREES && m.related)
+      corr = (float)nd_v / (float)nd_w;
+    const float acc_prob = corr * bbs_expf((prop_score - cur_score) * a.gamma);
+    const bool accept = acc_prob > raw.acceptance_level;
+    const bool new_max = (a.chain_offset + chain == 0u && step == 0u) ||
+                         prop_score > best_score;
+
+    if (TRACE && chain < a.n_traced && lane == 0u) {
+      bbs_step_trace tr;
+      tr.move_type = m.type;
+      tr.v = v;
+      tr.w = w;
+      tr.parent_of_v = parent_v;
+      tr.parent_of_w = parent_w;
+      tr.descendant_of_v = desc_v;
+      tr.nondescendant_of_v = nondesc_v;
+      tr.new_beta = nb;
+      tr.proposed_score = prop_score;
+      tr.flags = (accept ? 1u : 0u) | (new_max ? 2u : 0u);
+      a.trace[(uint64_t)chain * a.trace_stride + step] = tr;
+    }
+
+    if (new_max) {
+      best_score = prop_score;
+      best_beta = prop_beta;
+      best_step = step;
+#pragma unroll
+      for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) {
+          bam[k][wd] = pam[k][wd];
+          bdm[k][wd] = pdm[k][wd];
+        }
+    }
+    if (accept) {
+      cur_score = prop_score;
+      cur_beta = prop_beta;
+      L = Lp;
+      cur_prior = prop_prior;
+#pragma unroll
+      for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) {
+          am[k][wd] = pam[k][wd];
+          dm[k][wd] = pdm[k][wd];
+        }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+    if (x < N) {
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        a.am[2u * (row_base + x) + (uint32_t)wd] = am[k][wd];
+        a.dm[2u * (row_base + x) + (uint32_t)wd] = dm[k][wd];
+        a.best_am[2u * (row_base + x) + (uint32_t)wd] = bam[k][wd];
+        a.best_dm[2u * (row_base + x) + (uint32_t)wd] = bdm[k][wd];
+      }
+      if (NS == 1) {
+        a.am[2u * (row_base + x) + 1u] = 0ull;
+        a.dm[2u * (row_base + x) + 1u] = 0ull;
+        a.best_am[2u * (row_base + x) + 1u] = 0ull;
+        a.best_dm[2u * (row_base + x) + 1u] = 0ull;
+      }
+    }
+  }
+  if (lane == 0u) {
+    a.beta[chain] = cur_beta;
+    a.score[chain] = cur_score;
+    a.best_score[chain] = best_score;
+    a.best_beta[chain] = best_beta;
+    a.best_step[chain] = best_step;
+  }
+}
+
+/* ------------------------------------------------------------ score kernel */
+
+template <int NS, int CS, int W, bool COUNTS>
+__global__ __launch_bounds__(64) void score_kernel(ScoreArgs a) {
+  __shared__ __align__(16) float lds[64 * CS];
+  const uint32_t lane = lane_id();
+  const uint32_t N = a.n_nodes;
+  Cells<CS, W> cells;
+  load_cells<CS, W>(a.cells, a.n_cells, cells);
+  for (uint32_t tree = blockIdx.x; tree < a.n_trees; tree += gridDim.x) {
+    uint64_t dm[NS][NS];
+    const uint64_t base = (uint64_t)tree * N;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++)
+        dm[k][wd] = x < N ? a.dm[2u * (base + x) + (uint32_t)wd] : 0ull;
+    }
+    LogProbs L;
+    L.L00 = a.L00;
+    L.L10 = a.L10;
+    float prior;
+    if (a.betas) {
+      const float beta = a.betas[tree];
+      L.L01 = bbs_logf(beta);
+      L.L11 = bbs_logf_of_double(1.0 - (double)beta);
+      prior = logscore_beta(a.prior, beta);
+    } else {
+      L.L01 = a.L01;
+      L.L11 = a.L11;
+      prior = a.prior_value;
+    }
+    uint32_t arg01[CS], arg11[CS];
+    float T = score_tree<NS, CS, W, COUNTS>(dm, cells, N, a.n_cells, L, lds,
+                                            arg01, arg11);
+    if (lane == 0u) a.ll[tree] = a.with_prior ? T + prior : T;
+    if (COUNTS) {
+#pragma unroll
+      for (int j = 0; j < CS; j++) {
+        const uint32_t cell = lane + 64u * (uint32_t)j;
+        if (cell < a.n_cells) {
+          const uint32_t n01 = arg01[j], n11 = arg11[j];
+          const uint32_t n00 = (uint32_t)cells.fa[j] - n01;
+          const uint32_t n10 = (uint32_t)cells.fb[j] - n11;
+          const uint32_t packed = n00 | (n01 << 8) | (n10 << 16) | (n11 << 24);
+          reinterpret_cast<uint32_t *>(a.counts)[(uint64_t)tree * a.n_cells + cell] =
+              packed;
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------- random trees */
+
+/* Uniform random trees from Pruefer codes drawn from the Philox stream
+ * (purpose, chain = tree index + chain_offset); one wave per tree.
+ * Writes am (nullable) and dm rows [tree][n_nodes][2] and beta (nullable). */
+__global__ __launch_bounds__(64) void random_trees_kernel(TreeGenArgs a) {
+  __shared__ uint32_t degree[128];
+  __shared__ uint32_t parent[128];
+  __shared__ uint32_t code[128];
+  __shared__ uint64_t rows[128][2];
+  const uint32_t lane = lane_id();
+  const uint32_t N = a.n_nodes;
+  for (uint32_t tree = blockIdx.x; tree < a.n_trees; tree += gridDim.x) {
+    const uint32_t chain = a.chain_offset + tree;
+    /* code[i] ~ U{0..N-1}, i < N-2 (src/MutationTree.hpp:500-507) */
+    for (uint32_t i = lane; i + 2u < N; i += 64u) {
+      bbs_u32x4 b = bbs_philox_block(a.seed, a.job, chain, i >> 2, a.purpose);
+      code[i] = bbs_uint_below(b.x[i & 3u], N);
+    }
+    for (uint32_t i = lane; i < 128u; i += 64u) {
+      degree[i] = 1u;
+      parent[i] = N - 1u;
+    }
+    __syncthreads();
+    if (lane == 0u)
+      for (uint32_t i = 0; i + 2u < N; i++) degree[code[i]]++;
+    __syncthreads();
+    /* decode: repeatedly attach the smallest leaf (src/MutationTree.hpp:454-463) */
+    for (uint32_t i = 0; i + 2u < N; i++) {
+      bool leaf0 = lane < N && degree[lane] == 1u;
+      bool leaf1 = lane + 64u < N && degree[lane + 64u] == 1u;
+      uint64_t m0 = ballot(leaf0), m1 = ballot(leaf1);
+      uint32_t j = m0 ? (uint32_t)__ffsll((long long)m0) - 1u
+                      : 64u + (uint32_t)__ffsll((long long)m1) - 1u;
+      __syncthreads();
+      if (lane == 0u) {
+        uint32_t c = code[i];
+        parent[j] = c;
+        degree[c]--;
+        degree[j]--;
+      }
+      __syncthreads();
+    }
+    /* the remaining leaf other than the root hangs below the root: parent
+     * already initialised to N-1 (src/MutationTree.hpp:467-473) */
+    /* ancestor rows: dm[x] = {x} | dm[parent[x]], iterated to a fixed point */
+    uint64_t d[2][2];
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      d[k][0] = (x < N && x < 64u) ? (1ull << x) : 0ull;
+      d[k][1] = (x < N && x >= 64u) ? (1ull << (x - 64u)) : 0ull;
+      if (x < N) {
+        /* root bit */
+        const uint32_t r = N - 1u;
+        d[k][r >> 6] |= 1ull << (r & 63u);
+      }
+    }
+    for (uint32_t iter = 0; iter < N; iter++) {
+      for (int k = 0; k < 2; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+        rows[x][0] = d[k][0];
+        rows[x][1] = d[k][1];
+      }
+      __syncthreads();
+      bool changed = false;
+      for (int k = 0; k < 2; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+        if (x < N && x != N - 1u) {
+          const uint32_t p = parent[x];
+          const uint64_t n0 = d[k][0] | rows[p][0], n1 = d[k][1] | rows[p][1];
+          changed = changed || n0 != d[k][0] || n1 != d[k][1];
+          d[k][0] = n0;
+          d[k][1] = n1;
+        }
+      }
+      __syncthreads();
+      if (ballot(changed) == 0ull) break;
+    }
+    const uint64_t base = (uint64_t)tree * N;
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      if (x < N) {
+        a.dm[2u * (base + x)] = d[k][0];
+        a.dm[2u * (base + x) + 1u] = d[k][1];
+      }
+    }
+    if (a.am) {
+      /* am[x] bit y = dm[y] bit x: transpose through LDS */
+      for (int k = 0; k < 2; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+        rows[x][0] = d[k][0];
+        rows[x][1] = d[k][1];
+      }
+      __syncthreads();
+      for (int k = 0; k < 2; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+        uint64_t r0 = 0ull, r1 = 0ull;
+        if (x < N) {
+          for (uint32_t y = 0; y < N; y++) {
+            const bool bit = (rows[y][x >> 6] >> (x & 63u)) & 1u;
+            if (bit) {
+              if (y < 64u)
+                r0 |= 1ull << y;
+              else
+                r1 |= 1ull << (y - 64u);
+            }
+          }
+          a.am[2u * (base + x)] = r0;
+          a.am[2u * (base + x) + 1u] = r1;
+        }
+      }
+    }
+    if (a.beta && lane == 0u) a.beta[tree] = a.beta_value;
+    __syncthreads();
+  }
+}
+
+/* ------------------------------------------------------------ libm probe */
+
+/* Device values of the three libm restatements, for comparison with glibc. */
+__global__ __launch_bounds__(256) void libm_kernel(const float *x, uint32_t n,
+                                                   float *e, float *l, float *l1m) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float v = x[i];
+  e[i] = bbs_expf(v);
+  l[i] = bbs_logf(v);
+  l1m[i] = bbs_logf_of_double(1.0 - (double)v);
+}
+
+hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,
+                       hipStream_t stream) {
+  if (n == 0u) return hipSuccess;
+  hipLaunchKernelGGL(libm_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, x,
+                     n, e, l, l1m);
+  return hipGetLastError();
+}
+
+/* ------------------------------------------------------------ dispatch */
+
+#define BBS_SHAPES(X)                                                          \
+  X(1, 1, 1) X(1, 1, 2) X(1, 2, 1) X(1, 2, 2) X(2, 1, 2) X(2, 1, 3) X(2, 1, 4) \
+      X(2, 2, 2) X(2, 2, 3) X(2, 2, 4)
+
+void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w) {
+  *ns = (n_genes + 1u) <= 64u ? 1 : 2;
+  *cs = n_cells <= 64u ? 1 : 2;
+  *w = (int)((n_genes + 31u) / 32u);
+  if (*w < 1) *w = 1;
+  if (*ns == 2 && *w < 2) *w = 2;
+}
+
+hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream) {
+  int ns, cs, w;
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w);
+  dim3 grid(a.n_chains), block(64);
+#define X(NS_, CS_, W_)                                                        \
+  if (ns == NS_ && cs == CS_ && w == W_) {                                     \
+    if (trace)                                                                 \
+      hipLaunchKernelGGL((chain_kernel<NS_, CS_, W_, true>), grid, block, 0,   \
+                         stream, a);                                           \
+    else                                                                       \
+      hipLaunchKernelGGL((chain_kernel<NS_, CS_, W_, false>), grid, block, 0,  \
+                         stream, a);                                           \
+    return hipGetLastError();                                                  \
+  }
+  BBS_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream) {
+  int ns, cs, w;
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w);
+  uint32_t blocks = a.n_trees < 65536u ? a.n_trees : 65536u;
+  if (blocks == 0u) return hipSuccess;
+  dim3 grid(blocks), block(64);
+#define X(NS_, CS_, W_)                                                        \
+  if (ns == NS_ && cs == CS_ && w == W_) {                                     \
+    if (counts)                                                                \
+      hipLaunchKernelGGL((score_kernel<NS_, CS_, W_, true>), grid, block, 0,   \
+                         stream, a);                                           \
+    else                                                                       \
+      hipLaunchKernelGGL((score_kernel<NS_, CS_, W_, false>), grid, block, 0,  \
+                         stream, a);                                           \
+    return hipGetLastError();                                                  \
+  }
+  BBS_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream) {
+  uint32_t blocks = a.n_trees < 65536u ? a.n_trees : 65536u;
+  if (blocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(random_trees_kernel, dim3(blocks), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+} // namespace bbs

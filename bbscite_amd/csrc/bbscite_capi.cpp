@@ -1,0 +1,967 @@
+/*
+ * C-ABI of the MI355X chain search (include/bbscite.h): context lifetime,
+ * device memory, kernel launches and the host-side helpers (CSV parsing,
+ * encoding, synthetic data, Pruefer decoding, Graphviz/Newick text).
+ *
+ * Reference counterparts are named on each entry point in the header; the
+ * host-side restatements here follow src/main.cpp:52-144,
+ * src/Parameters.hpp:39-45,451-612, src/TreeScorer.hpp:92-126 and
+ * src/MutationTree.hpp:379-510,549-596,799-894.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bbscite.h"
+#include "bbs_kernels.h"
+#include "bbs_libm.h"
+#include "bbs_rng.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+} // namespace
+
+struct bbs_ctx {
+  bbs_params params;
+  uint32_t n_cells = 0, n_genes = 0, n_nodes = 0;
+  float L00 = 0.f, L10 = 0.f;
+  bbs::PriorArgs prior{};
+  bbs_move_probs probs{};
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf cells, am, dm, beta, score, best_am, best_dm, best_beta, best_score,
+      best_step, trace;
+  uint32_t n_traced = 0;
+  bool states_set = false;   /* initial states present on the device */
+  bool score_known = false;  /* initial scores present on the device  */
+  bool begun = false;        /* bests reset, steps counted from 0     */
+  uint32_t steps_done = 0;
+  bool ran = false;
+  std::string err;
+};
+
+namespace {
+
+int fail(bbs_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx)
+    ctx->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail((ctx), BBS_ERR_HIP, "%s failed: %s", #expr,                  \
+                  hipGetErrorString(e_));                                      \
+  } while (0)
+
+int dev_alloc(bbs_ctx *ctx, DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return BBS_OK;
+  b.release();
+  if (bytes == 0) return BBS_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(ctx, e == hipErrorOutOfMemory ? BBS_ERR_BUDGET : BBS_ERR_HIP,
+                "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return BBS_OK;
+}
+
+/* log(tgamma(x)) as the reference computes it in float
+ * (src/TreeScorer.hpp:123-125). Where tgammaf overflows (x > ~35.04) the
+ * reference's prior is inf - inf = NaN; this build uses log-gamma in double
+ * instead (the only deliberate numeric deviation; same rule in the oracle). */
+float lgamma_term(float x) {
+  float g = tgammaf(x);
+  if (std::isinf(g) && g > 0 && x > 0) return (float)lgamma((double)x);
+  return logf(g);
+}
+
+bool is_gfx950(int dev) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+/* ---------------- host tree helpers ---------------- */
+
+int pruefer_to_parent(const uint32_t *code, uint32_t len, uint32_t *parent) {
+  uint32_t n = len + 2u;
+  if (n > BBS_MAX_NODES) return BBS_ERR_SIZE;
+  uint32_t degree[BBS_MAX_NODES];
+  for (uint32_t i = 0; i < n; i++) degree[i] = 1u;
+  for (uint32_t i = 0; i < len; i++) {
+    if (code[i] >= n) return BBS_ERR_ARG;
+    degree[code[i]]++;
+  }
+  for (uint32_t i = 0; i < n; i++) parent[i] = n - 1u;
+  /* attach the smallest remaining leaf to the next code entry
+   * (src/MutationTree.hpp:454-463) */
+  for (uint32_t i = 0; i < len; i++) {
+    for (uint32_t j = 0; j < n; j++) {
+      if (degree[j] == 1u) {
+        parent[j] = code[i];
+        degree[code[i]]--;
+        degree[j]--;
+        break;
+      }
+    }
+  }
+  return BBS_OK;
+}
+
+int parent_to_rows(const uint32_t *parent, uint32_t n, uint64_t *am,
+                   uint64_t *dm) {
+  if (n < 1u || n > BBS_MAX_NODES) return BBS_ERR_SIZE;
+  const uint32_t root = n - 1u;
+  if (am) std::memset(am, 0, sizeof(uint64_t) * 2u * n);
+  std::vector<uint64_t> d(2u * n, 0ull);
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t x = i, guard = 0;
+    while (x != root) {
+      if (x >= n || guard++ > n) return BBS_ERR_ARG; /* not a tree */
+      d[2u * i + (x >> 6)] |= 1ull << (x & 63u);
+      x = parent[x];
+    }
+    d[2u * i + (root >> 6)] |= 1ull << (root & 63u);
+  }
+  if (dm) std::memcpy(dm, d.data(), sizeof(uint64_t) * 2u * n);
+  if (am)
+    for (uint32_t y = 0; y < n; y++)
+      for (uint32_t x = 0; x < n; x++)
+        if ((d[2u * y + (x >> 6)] >> (x & 63u)) & 1u)
+          am[2u * x + (y >> 6)] |= 1ull << (y & 63u);
+  return BBS_OK;
+}
+
+inline bool rbit(const uint64_t *row, uint32_t i) {
+  return (row[i >> 6] >> (i & 63u)) & 1u;
+}
+
+/* get_parent of src/MutationTree.hpp:549-596 on dm rows. */
+uint32_t rows_parent(const uint64_t *dm, uint32_t n, uint32_t child) {
+  uint32_t parent = 0;
+  for (uint32_t p = 0; p < n; p++) {
+    const uint64_t *pa = dm + 2u * p, *ca = dm + 2u * child;
+    if (!rbit(ca, p)) continue;
+    if (p == child) {
+      if (child == n - 1u) parent = p;
+      continue;
+    }
+    bool same = true;
+    for (uint32_t i = 0; i < n && same; i++)
+      if (i != child && rbit(pa, i) != rbit(ca, i)) same = false;
+    if (same) parent = p;
+  }
+  return parent;
+}
+
+struct TextOut {
+  char *buf;
+  uint64_t cap, len;
+  void put(const std::string &s) {
+    for (char c : s) {
+      if (len + 1u < cap) buf[len] = c;
+      len++;
+    }
+  }
+  void finish() {
+    if (cap) buf[len + 1u < cap ? len : cap - 1u] = 0;
+  }
+};
+
+void newick_rec(const std::vector<std::vector<uint32_t>> &children, uint32_t node,
+                TextOut &out, uint32_t depth) {
+  if (depth > BBS_MAX_NODES) return;
+  const auto &ch = children[node];
+  if (!ch.empty()) {
+    out.put("(");
+    for (size_t i = 0; i < ch.size(); i++) {
+      newick_rec(children, ch[i], out, depth + 1u);
+      if (i + 1u != ch.size()) out.put(",");
+    }
+    out.put(")");
+  }
+  out.put(std::to_string(node));
+}
+
+void gen_tree_parent(uint64_t seed, uint32_t job, uint32_t chain,
+                     uint32_t purpose, uint32_t n, uint32_t *parent) {
+  uint32_t code[BBS_MAX_NODES];
+  for (uint32_t i = 0; i + 2u < n; i++) {
+    bbs_u32x4 b = bbs_philox_block(seed, job, chain, i >> 2, purpose);
+    code[i] = bbs_uint_below(b.x[i & 3u], n);
+  }
+  pruefer_to_parent(code, n - 2u, parent);
+}
+
+} // namespace
+
+/* ======================================================================= */
+
+extern "C" {
+
+void bbs_params_default(bbs_params *p) {
+  if (!p) return;
+  /* src/Parameters.hpp:39-45 */
+  p->alpha_mean = 6.04e-5f;
+  p->beta_mean = 0.4309f;
+  p->beta_sd = 0.1f;
+  p->gamma = 1.0f;
+  p->prob_beta_change = 0.0f;
+  p->prob_prune_n_reattach = 0.55f;
+  p->prob_swap_nodes = 0.4f;
+  p->prob_swap_subtrees = 0.05f;
+  p->beta_jump_scaling_chi = 10.0f;
+  p->n_chains = 0;
+  p->n_steps = 0;
+  p->seed = 0;
+}
+
+const char *bbs_last_error(const bbs_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
+               const uint64_t *is_mutated, const uint64_t *is_known,
+               bbs_ctx **out) {
+  if (!out) return fail(nullptr, BBS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!params || !is_mutated || !is_known)
+    return fail(nullptr, BBS_ERR_ARG, "params and data must not be NULL");
+  if (n_genes < 2u || n_genes > BBS_MAX_GENES)
+    return fail(nullptr, BBS_ERR_SIZE,
+                "this build supports 2 to %u genes; the input has %u",
+                BBS_MAX_GENES, n_genes);
+  if (n_cells < 1u || n_cells > BBS_MAX_CELLS)
+    return fail(nullptr, BBS_ERR_SIZE,
+                "this build supports 1 to %u cells; the input has %u",
+                BBS_MAX_CELLS, n_cells);
+  const bbs_params &p = *params;
+  if (!(p.alpha_mean > 0.f && p.alpha_mean < 1.f))
+    return fail(nullptr, BBS_ERR_ARG, "alpha (-fd) must lie in (0, 1)");
+  if (!(p.beta_mean > 0.f && p.beta_mean < 1.f))
+    return fail(nullptr, BBS_ERR_ARG, "beta (-ad) must lie in (0, 1)");
+  if (!(p.beta_sd > 0.f))
+    return fail(nullptr, BBS_ERR_ARG, "beta sd (-sd) must be positive");
+  if (!(p.prob_beta_change >= 0.f && p.prob_prune_n_reattach >= 0.f &&
+        p.prob_swap_nodes >= 0.f && p.prob_swap_subtrees >= 0.f))
+    return fail(nullptr, BBS_ERR_ARG, "Move probabilities may not be negative.");
+  if (p.prob_beta_change + p.prob_prune_n_reattach + p.prob_swap_nodes +
+          p.prob_swap_subtrees ==
+      0.f)
+    return fail(nullptr, BBS_ERR_ARG,
+                "The sum of the move type probabilities is zero.");
+  if (!(p.beta_jump_scaling_chi > 0.f))
+    return fail(nullptr, BBS_ERR_ARG, "chi (-x) must be positive");
+
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
+    return fail(nullptr, BBS_ERR_NODEV, "no HIP device available");
+  int dev = -1;
+  for (int d = 0; d < n_dev && dev < 0; d++)
+    if (is_gfx950(d)) dev = d;
+  if (dev < 0) return fail(nullptr, BBS_ERR_NODEV, "no gfx950 device found");
+
+  bbs_ctx *ctx = new bbs_ctx();
+  ctx->params = p;
+  ctx->n_cells = n_cells;
+  ctx->n_genes = n_genes;
+  ctx->n_nodes = n_genes + 1u;
+  ctx->device = dev;
+
+  /* TreeScorer constructor (src/TreeScorer.hpp:92-117) */
+  ctx->L00 = (float)std::log(1.0 - (double)p.alpha_mean);
+  ctx->L10 = logf(p.alpha_mean);
+  double a = ((double)(1.0f - p.beta_mean) * std::pow((double)p.beta_mean, 2.0) /
+              std::pow((double)p.beta_sd, 2.0)) -
+             (double)p.beta_mean;
+  float bpa = (float)a;
+  float bpb = bpa * ((1.0f / p.beta_mean) - 1.0f);
+  ctx->prior.am1 = bpa - 1.0f;
+  ctx->prior.bm1 = bpb - 1.0f;
+  ctx->prior.lg_ab = lgamma_term(bpa + bpb);
+  ctx->prior.lg_a = lgamma_term(bpa);
+  ctx->prior.lg_b = lgamma_term(bpb);
+
+  /* normalised move probabilities (src/Parameters.hpp:451-541,609-612) */
+  float sum = p.prob_beta_change + p.prob_prune_n_reattach + p.prob_swap_nodes +
+              p.prob_swap_subtrees;
+  float pb = p.prob_beta_change / sum, pp = p.prob_prune_n_reattach / sum,
+        ps = p.prob_swap_nodes / sum;
+  ctx->probs.t_beta = pb;
+  ctx->probs.t_prune = pb + pp;
+  ctx->probs.t_swap = pb + pp + ps;
+  ctx->probs.beta_jump_sd = p.beta_sd / p.beta_jump_scaling_chi;
+
+  /* per-cell K&M and K&~M restricted to the genes (src/main.cpp:103-144) */
+  std::vector<uint64_t> cells(4u * n_cells);
+  uint64_t gmask[2] = {n_genes >= 64u ? ~0ull : ((1ull << n_genes) - 1ull),
+                       n_genes <= 64u ? 0ull : ((1ull << (n_genes - 64u)) - 1ull)};
+  for (uint32_t c = 0; c < n_cells; c++) {
+    for (int wd = 0; wd < 2; wd++) {
+      uint64_t K = is_known[2u * c + wd] & gmask[wd];
+      uint64_t M = is_mutated[2u * c + wd];
+      cells[4u * c + wd] = K & M;
+      cells[4u * c + 2u + wd] = K & ~M;
+    }
+  }
+
+  int rc;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e != hipSuccess) {
+    g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
+    bbs_destroy(ctx);
+    return BBS_ERR_HIP;
+  }
+  const size_t rows = (size_t)p.n_chains * ctx->n_nodes * 2u * sizeof(uint64_t);
+  const size_t per_chain = (size_t)p.n_chains * sizeof(float);
+  if ((rc = dev_alloc(ctx, ctx->cells, cells.size() * sizeof(uint64_t))) ||
+      (rc = dev_alloc(ctx, ctx->am, rows)) || (rc = dev_alloc(ctx, ctx->dm, rows)) ||
+      (rc = dev_alloc(ctx, ctx->best_am, rows)) ||
+      (rc = dev_alloc(ctx, ctx->best_dm, rows)) ||
+      (rc = dev_alloc(ctx, ctx->beta, per_chain)) ||
+      (rc = dev_alloc(ctx, ctx->score, per_chain)) ||
+      (rc = dev_alloc(ctx, ctx->best_beta, per_chain)) ||
+      (rc = dev_alloc(ctx, ctx->best_score, per_chain)) ||
+      (rc = dev_alloc(ctx, ctx->best_step, per_chain))) {
+    g_create_error = ctx->err;
+    bbs_destroy(ctx);
+    return rc;
+  }
+  e = hipMemcpy(ctx->cells.p, cells.data(), cells.size() * sizeof(uint64_t),
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    g_create_error = std::string("hipMemcpy failed: ") + hipGetErrorString(e);
+    bbs_destroy(ctx);
+    return BBS_ERR_HIP;
+  }
+  *out = ctx;
+  return BBS_OK;
+}
+
+void bbs_destroy(bbs_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->cells,     &ctx->am,         &ctx->dm,
+                    &ctx->beta,      &ctx->score,      &ctx->best_am,
+                    &ctx->best_dm,   &ctx->best_beta,  &ctx->best_score,
+                    &ctx->best_step, &ctx->trace};
+  for (DevBuf *b : bufs) b->release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int bbs_set_initial_states(bbs_ctx *ctx, const uint64_t *am, const uint64_t *dm,
+                           const float *beta, const float *score) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!am || !dm || !beta)
+    return fail(ctx, BBS_ERR_ARG, "am, dm and beta must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t rows = (size_t)ctx->params.n_chains * ctx->n_nodes * 2u * sizeof(uint64_t);
+  const size_t per_chain = (size_t)ctx->params.n_chains * sizeof(float);
+  HIP_TRY(ctx, hipMemcpy(ctx->am.p, am, rows, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(ctx->dm.p, dm, rows, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(ctx->beta.p, beta, per_chain, hipMemcpyHostToDevice));
+  if (score)
+    HIP_TRY(ctx, hipMemcpy(ctx->score.p, score, per_chain, hipMemcpyHostToDevice));
+  ctx->states_set = true;
+  ctx->score_known = score != nullptr;
+  ctx->begun = false;
+  ctx->ran = false;
+  return BBS_OK;
+}
+
+int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (n_traced_chains > ctx->params.n_chains)
+    return fail(ctx, BBS_ERR_ARG, "cannot trace %u of %u chains", n_traced_chains,
+                ctx->params.n_chains);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = dev_alloc(ctx, ctx->trace,
+                     (size_t)n_traced_chains * ctx->params.n_steps *
+                         sizeof(bbs_step_trace));
+  if (rc) return rc;
+  ctx->n_traced = n_traced_chains;
+  return BBS_OK;
+}
+
+/* Start a run: initial states (drawn from the seed unless set), no bests. */
+int bbs_begin(bbs_ctx *ctx) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const uint32_t R = ctx->params.n_chains;
+  if (!ctx->states_set) {
+    bbs::TreeGenArgs g{};
+    g.am = (uint64_t *)ctx->am.p;
+    g.dm = (uint64_t *)ctx->dm.p;
+    g.beta = (float *)ctx->beta.p;
+    g.beta_value = ctx->params.beta_mean;
+    g.seed = ctx->params.seed;
+    g.job = 0u;
+    g.chain_offset = 0u;
+    g.purpose = BBS_PURPOSE_INIT_TREE;
+    g.n_trees = R;
+    g.n_nodes = ctx->n_nodes;
+    HIP_TRY(ctx, bbs::launch_random_trees(g, ctx->stream));
+    ctx->states_set = true;
+    ctx->score_known = false;
+  }
+  std::vector<uint32_t> steps(R, 0xffffffffu);
+  std::vector<float> scores(R, -INFINITY);
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->best_step.p, steps.data(), R * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->best_score.p, scores.data(), R * sizeof(float),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->best_beta.p, scores.data(), R * sizeof(float),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->begun = true;
+  ctx->steps_done = 0;
+  ctx->ran = false;
+  return BBS_OK;
+}
+
+/* Advance every chain by n steps (continuing the current run). */
+int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!ctx->begun) {
+    int rc = bbs_begin(ctx);
+    if (rc) return rc;
+  }
+  if ((uint64_t)ctx->steps_done + n > ctx->params.n_steps)
+    return fail(ctx, BBS_ERR_ARG, "advance beyond n_steps (%u + %u > %u)",
+                ctx->steps_done, n, ctx->params.n_steps);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  bbs::ChainArgs a{};
+  a.cells = (const uint64_t *)ctx->cells.p;
+  a.am = (uint64_t *)ctx->am.p;
+  a.dm = (uint64_t *)ctx->dm.p;
+  a.beta = (float *)ctx->beta.p;
+  a.score = (float *)ctx->score.p;
+  a.best_am = (uint64_t *)ctx->best_am.p;
+  a.best_dm = (uint64_t *)ctx->best_dm.p;
+  a.best_beta = (float *)ctx->best_beta.p;
+  a.best_score = (float *)ctx->best_score.p;
+  a.best_step = (uint32_t *)ctx->best_step.p;
+  a.trace = (bbs_step_trace *)ctx->trace.p;
+  a.n_traced = ctx->n_traced;
+  a.trace_stride = ctx->params.n_steps;
+  a.seed = ctx->params.seed;
+  a.job = 0u;
+  a.chain_offset = 0u;
+  a.n_chains = ctx->params.n_chains;
+  a.step_begin = ctx->steps_done;
+  a.step_end = ctx->steps_done + n;
+  a.n_cells = ctx->n_cells;
+  a.n_nodes = ctx->n_nodes;
+  a.score_init = (ctx->steps_done == 0u && !ctx->score_known) ? 1u : 0u;
+  a.L00 = ctx->L00;
+  a.L10 = ctx->L10;
+  a.gamma = ctx->params.gamma;
+  a.prior = ctx->prior;
+  a.probs = ctx->probs;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (a.n_chains > 0u)
+    HIP_TRY(ctx, bbs::launch_chain(a, ctx->n_traced > 0u, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (makespan_ms) *makespan_ms = ms;
+  ctx->steps_done += n;
+  ctx->score_known = true;
+  ctx->ran = ctx->steps_done == ctx->params.n_steps;
+  return BBS_OK;
+}
+
+int bbs_run(bbs_ctx *ctx, float *makespan_ms) {
+  int rc = bbs_begin(ctx);
+  if (rc) return rc;
+  return bbs_advance(ctx, ctx->params.n_steps, makespan_ms);
+}
+
+int bbs_get_chain_results(bbs_ctx *ctx, float *best_score, uint32_t *best_step,
+                          float *final_score, float *final_beta) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!ctx->begun) return fail(ctx, BBS_ERR_STATE, "no run yet");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t n = ctx->params.n_chains;
+  if (best_score)
+    HIP_TRY(ctx, hipMemcpy(best_score, ctx->best_score.p, n * 4u, hipMemcpyDeviceToHost));
+  if (best_step)
+    HIP_TRY(ctx, hipMemcpy(best_step, ctx->best_step.p, n * 4u, hipMemcpyDeviceToHost));
+  if (final_score)
+    HIP_TRY(ctx, hipMemcpy(final_score, ctx->score.p, n * 4u, hipMemcpyDeviceToHost));
+  if (final_beta)
+    HIP_TRY(ctx, hipMemcpy(final_beta, ctx->beta.p, n * 4u, hipMemcpyDeviceToHost));
+  return BBS_OK;
+}
+
+int bbs_get_best(bbs_ctx *ctx, uint64_t *am, uint64_t *dm, float *beta,
+                 float *score) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!ctx->begun || ctx->steps_done == 0u)
+    return fail(ctx, BBS_ERR_STATE, "no steps have run");
+  const uint32_t R = ctx->params.n_chains;
+  std::vector<float> s(R), b(R);
+  std::vector<uint32_t> st(R);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpy(s.data(), ctx->best_score.p, R * 4u, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(st.data(), ctx->best_step.p, R * 4u, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(b.data(), ctx->best_beta.p, R * 4u, hipMemcpyDeviceToHost));
+  /* Chain-major order (src/Application.hpp:408-456, P = 1): chain 0's first
+   * proposal seeds the maximum; a later chain replaces it only with a
+   * strictly greater proposal, its first such one. */
+  uint32_t best = 0u;
+  for (uint32_t c = 1; c < R; c++)
+    if (st[c] != 0xffffffffu && s[c] > s[best]) best = c;
+  const size_t row_bytes = (size_t)ctx->n_nodes * 2u * sizeof(uint64_t);
+  if (am)
+    HIP_TRY(ctx, hipMemcpy(am, (const char *)ctx->best_am.p + best * row_bytes,
+                           row_bytes, hipMemcpyDeviceToHost));
+  if (dm)
+    HIP_TRY(ctx, hipMemcpy(dm, (const char *)ctx->best_dm.p + best * row_bytes,
+                           row_bytes, hipMemcpyDeviceToHost));
+  if (beta) *beta = b[best];
+  if (score) *score = s[best];
+  return BBS_OK;
+}
+
+int bbs_get_chain_states(bbs_ctx *ctx, uint64_t *am, uint64_t *dm) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!ctx->states_set) return fail(ctx, BBS_ERR_STATE, "no chain states yet");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t rows = (size_t)ctx->params.n_chains * ctx->n_nodes * 2u * sizeof(uint64_t);
+  if (am) HIP_TRY(ctx, hipMemcpy(am, ctx->am.p, rows, hipMemcpyDeviceToHost));
+  if (dm) HIP_TRY(ctx, hipMemcpy(dm, ctx->dm.p, rows, hipMemcpyDeviceToHost));
+  return BBS_OK;
+}
+
+int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!out) return fail(ctx, BBS_ERR_ARG, "out is NULL");
+  if (!ctx->ran) return fail(ctx, BBS_ERR_STATE, "run not finished");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpy(out, ctx->trace.p,
+                         (size_t)ctx->n_traced * ctx->params.n_steps *
+                             sizeof(bbs_step_trace),
+                         hipMemcpyDeviceToHost));
+  return BBS_OK;
+}
+
+/* Fill a ScoreArgs for this context; betas == nullptr means beta_mean. */
+static void score_args(bbs_ctx *ctx, bbs::ScoreArgs &s, uint32_t flags) {
+  s.cells = (const uint64_t *)ctx->cells.p;
+  s.n_cells = ctx->n_cells;
+  s.n_nodes = ctx->n_nodes;
+  s.with_prior = (flags & BBS_SCORE_NO_PRIOR) ? 0u : 1u;
+  s.L00 = ctx->L00;
+  s.L10 = ctx->L10;
+  const float b = ctx->params.beta_mean;
+  s.L01 = bbs_logf(b);
+  s.L11 = bbs_logf_of_double(1.0 - (double)b);
+  s.prior = ctx->prior;
+  s.prior_value = ctx->prior.lg_ab + ctx->prior.am1 * bbs_logf(b) +
+                  ctx->prior.bm1 * bbs_logf(1.0f - b) - ctx->prior.lg_a -
+                  ctx->prior.lg_b;
+}
+
+int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
+                    const float *betas, uint32_t flags, float *ll,
+                    uint8_t *counts) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (n_trees == 0u) return BBS_OK;
+  if (!dm_rows || !ll) return fail(ctx, BBS_ERR_ARG, "dm_rows and ll must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t rows = (size_t)n_trees * ctx->n_nodes * 2u * sizeof(uint64_t);
+  DevBuf d_rows, d_betas, d_ll, d_counts;
+  int rc = dev_alloc(ctx, d_rows, rows);
+  if (!rc && betas) rc = dev_alloc(ctx, d_betas, (size_t)n_trees * 4u);
+  if (!rc) rc = dev_alloc(ctx, d_ll, (size_t)n_trees * 4u);
+  if (!rc && counts) rc = dev_alloc(ctx, d_counts, (size_t)n_trees * ctx->n_cells * 4u);
+  if (rc) {
+    d_rows.release();
+    d_betas.release();
+    d_ll.release();
+    d_counts.release();
+    return rc;
+  }
+  bbs::ScoreArgs s{};
+  score_args(ctx, s, flags);
+  s.dm = (const uint64_t *)d_rows.p;
+  s.betas = betas ? (const float *)d_betas.p : nullptr;
+  s.ll = (float *)d_ll.p;
+  s.counts = counts ? (uint8_t *)d_counts.p : nullptr;
+  s.n_trees = n_trees;
+  hipError_t e = hipMemcpy(d_rows.p, dm_rows, rows, hipMemcpyHostToDevice);
+  if (e == hipSuccess && betas)
+    e = hipMemcpy(d_betas.p, betas, (size_t)n_trees * 4u, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = bbs::launch_score(s, counts != nullptr, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(ll, d_ll.p, (size_t)n_trees * 4u, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && counts)
+    e = hipMemcpy(counts, d_counts.p, (size_t)n_trees * ctx->n_cells * 4u,
+                  hipMemcpyDeviceToHost);
+  d_rows.release();
+  d_betas.release();
+  d_ll.release();
+  d_counts.release();
+  if (e != hipSuccess)
+    return fail(ctx, BBS_ERR_HIP, "bbs_score_trees: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+/* Device-resident bulk scoring of n_trees uniform random trees (drawn on
+ * the device from seed), reps timed launches. kernel_ms[reps] receives each
+ * launch's HIP-event time on the launch stream; ll_head receives the first
+ * n_head scores (for checking against host-generated trees). */
+int bbs_bench_score_random(bbs_ctx *ctx, uint64_t seed, uint32_t n_trees,
+                           uint32_t reps, uint32_t flags, float *kernel_ms,
+                           float *ll_head, uint32_t n_head) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (n_head > n_trees) return fail(ctx, BBS_ERR_ARG, "n_head > n_trees");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  DevBuf d_rows, d_ll;
+  int rc = dev_alloc(ctx, d_rows, (size_t)n_trees * ctx->n_nodes * 16u);
+  if (!rc) rc = dev_alloc(ctx, d_ll, (size_t)n_trees * 4u);
+  if (rc) {
+    d_rows.release();
+    d_ll.release();
+    return rc;
+  }
+  bbs::TreeGenArgs g{};
+  g.dm = (uint64_t *)d_rows.p;
+  g.seed = seed;
+  g.purpose = BBS_PURPOSE_TREE_SAMPLE;
+  g.n_trees = n_trees;
+  g.n_nodes = ctx->n_nodes;
+  bbs::ScoreArgs s{};
+  score_args(ctx, s, flags);
+  s.dm = (const uint64_t *)d_rows.p;
+  s.ll = (float *)d_ll.p;
+  s.n_trees = n_trees;
+  hipError_t e = bbs::launch_random_trees(g, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  for (uint32_t r = 0; r < reps && e == hipSuccess; r++) {
+    e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = bbs::launch_score(s, false, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (kernel_ms) kernel_ms[r] = ms;
+  }
+  if (e == hipSuccess && n_head && ll_head)
+    e = hipMemcpy(ll_head, d_ll.p, (size_t)n_head * 4u, hipMemcpyDeviceToHost);
+  d_rows.release();
+  d_ll.release();
+  if (e != hipSuccess)
+    return fail(ctx, BBS_ERR_HIP, "bbs_bench_score_random: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+/* Device-generated random trees copied to the host (dm rows; am nullable). */
+int bbs_random_trees_device(bbs_ctx *ctx, uint64_t seed, uint32_t purpose,
+                            uint32_t n_trees, uint64_t *am, uint64_t *dm) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)n_trees * ctx->n_nodes * 16u;
+  DevBuf d_am, d_dm;
+  int rc = dev_alloc(ctx, d_dm, bytes);
+  if (!rc && am) rc = dev_alloc(ctx, d_am, bytes);
+  if (rc) {
+    d_am.release();
+    d_dm.release();
+    return rc;
+  }
+  bbs::TreeGenArgs g{};
+  g.am = am ? (uint64_t *)d_am.p : nullptr;
+  g.dm = (uint64_t *)d_dm.p;
+  g.seed = seed;
+  g.purpose = purpose;
+  g.n_trees = n_trees;
+  g.n_nodes = ctx->n_nodes;
+  hipError_t e = bbs::launch_random_trees(g, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && dm) e = hipMemcpy(dm, d_dm.p, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && am) e = hipMemcpy(am, d_am.p, bytes, hipMemcpyDeviceToHost);
+  d_am.release();
+  d_dm.release();
+  if (e != hipSuccess)
+    return fail(ctx, BBS_ERR_HIP, "bbs_random_trees_device: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
+                  float *log_out, float *log1m_out) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!x || !exp_out || !log_out || !log1m_out)
+    return fail(ctx, BBS_ERR_ARG, "NULL argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  DevBuf dx, de, dl, dm;
+  const size_t bytes = (size_t)n * 4u;
+  int rc = dev_alloc(ctx, dx, bytes);
+  if (!rc) rc = dev_alloc(ctx, de, bytes);
+  if (!rc) rc = dev_alloc(ctx, dl, bytes);
+  if (!rc) rc = dev_alloc(ctx, dm, bytes);
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = bbs::launch_libm((const float *)dx.p, n, (float *)de.p, (float *)dl.p,
+                           (float *)dm.p, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(exp_out, de.p, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(log_out, dl.p, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(log1m_out, dm.p, bytes, hipMemcpyDeviceToHost);
+  }
+  dx.release();
+  de.release();
+  dl.release();
+  dm.release();
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, BBS_ERR_HIP, "bbs_eval_libm: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+/* ---------------------------------------------------------------- host */
+
+int bbs_parse_csv(const char *text, uint64_t length, int8_t *matrix,
+                  uint32_t *n_genes_out, uint32_t *n_cells_out) {
+  if (!text || !matrix || !n_genes_out || !n_cells_out)
+    return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  /* pass 1: count and validate (src/main.cpp:52-101) */
+  uint32_t n_cells = 0, n_genes = 0, in_line = 0;
+  for (uint64_t i = 0; i < length; i++) {
+    char c = text[i];
+    if (c == '0' || c == '1' || c == '2' || c == '3') {
+      in_line++;
+    } else if (c == '\n') {
+      if (n_cells == 0u)
+        n_cells = in_line;
+      else if (n_cells != in_line)
+        return fail(nullptr, BBS_ERR_ARG, "Inconsistent rows in the input file!");
+      n_genes++;
+      in_line = 0;
+    } else if (c == ' ') {
+      continue;
+    } else {
+      return fail(nullptr, BBS_ERR_ARG, "Illegal character %c in the input file!", c);
+    }
+  }
+  if (in_line == n_cells)
+    n_genes++;
+  else if (in_line != 0u)
+    return fail(nullptr, BBS_ERR_ARG, "Inconsistent rows in the input file!");
+  if (n_cells == 0u || n_genes == 0u)
+    return fail(nullptr, BBS_ERR_ARG, "Empty input file!");
+  if (n_cells > BBS_MAX_CELLS)
+    return fail(nullptr, BBS_ERR_SIZE,
+                "This build of bbSCITE only supports up %u cells. The input data "
+                "contains %u cells.",
+                BBS_MAX_CELLS, n_cells);
+  if (n_genes > BBS_MAX_GENES)
+    return fail(nullptr, BBS_ERR_SIZE,
+                "This build of bbSCITE only supports up %u genes. The input data "
+                "contains %u genes.",
+                BBS_MAX_GENES, n_genes);
+  /* pass 2: whitespace-separated integers (src/main.cpp:120-143) */
+  uint64_t i = 0;
+  for (uint32_t g = 0; g < n_genes; g++) {
+    for (uint32_t c = 0; c < n_cells; c++) {
+      while (i < length && (text[i] == ' ' || text[i] == '\n')) i++;
+      long v = 0;
+      uint64_t start = i;
+      while (i < length && text[i] >= '0' && text[i] <= '9') {
+        v = v * 10 + (text[i] - '0');
+        if (v > 1000) v = 1000;
+        i++;
+      }
+      if (i == start || v > 3)
+        return fail(nullptr, BBS_ERR_ARG,
+                    "The input file contains the invalid entry %ld", v);
+      matrix[(size_t)g * n_cells + c] = (int8_t)v;
+    }
+  }
+  *n_genes_out = n_genes;
+  *n_cells_out = n_cells;
+  return BBS_OK;
+}
+
+int bbs_encode_matrix(const int8_t *matrix, uint32_t n_genes, uint32_t n_cells,
+                      uint64_t *is_mutated, uint64_t *is_known) {
+  if (!matrix || !is_mutated || !is_known)
+    return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  if (n_genes > BBS_MAX_GENES || n_cells > BBS_MAX_CELLS)
+    return fail(nullptr, BBS_ERR_SIZE, "matrix too large");
+  std::memset(is_mutated, 0, sizeof(uint64_t) * 2u * n_cells);
+  std::memset(is_known, 0, sizeof(uint64_t) * 2u * n_cells);
+  for (uint32_t g = 0; g < n_genes; g++) {
+    for (uint32_t c = 0; c < n_cells; c++) {
+      const uint64_t bit = 1ull << (g & 63u);
+      switch (matrix[(size_t)g * n_cells + c]) {
+      case 0:
+        is_known[2u * c + (g >> 6)] |= bit;
+        break;
+      case 1:
+      case 2:
+        is_known[2u * c + (g >> 6)] |= bit;
+        is_mutated[2u * c + (g >> 6)] |= bit;
+        break;
+      case 3:
+        break;
+      default:
+        return fail(nullptr, BBS_ERR_ARG,
+                    "The input file contains the invalid entry %d",
+                    (int)matrix[(size_t)g * n_cells + c]);
+      }
+    }
+  }
+  return BBS_OK;
+}
+
+int bbs_generate_dataset(uint64_t seed, uint32_t n_genes, uint32_t n_cells,
+                         float prob_false_positive, float prob_false_negative,
+                         float prob_missing, int8_t *matrix,
+                         uint32_t *true_parent, uint32_t *attachment) {
+  if (!matrix) return fail(nullptr, BBS_ERR_ARG, "matrix is NULL");
+  if (n_genes < 2u || n_genes > BBS_MAX_GENES || n_cells < 1u ||
+      n_cells > BBS_MAX_CELLS)
+    return fail(nullptr, BBS_ERR_SIZE, "unsupported size %u x %u", n_genes, n_cells);
+  const uint32_t n = n_genes + 1u;
+  uint32_t parent[BBS_MAX_NODES];
+  gen_tree_parent(seed, 0u, 0u, BBS_PURPOSE_DATA_TREE, n, parent);
+  std::vector<uint64_t> am(2u * n);
+  int rc = parent_to_rows(parent, n, am.data(), nullptr);
+  if (rc) return rc;
+  std::vector<uint32_t> att(n_cells);
+  for (uint32_t c = 0; c < n_cells; c++) {
+    bbs_u32x4 b = bbs_philox_block(seed, 0u, 0u, c >> 2, BBS_PURPOSE_DATA_CELL);
+    att[c] = bbs_uint_below(b.x[c & 3u], n);
+  }
+  for (uint32_t g = 0; g < n_genes; g++) {
+    for (uint32_t c = 0; c < n_cells; c++) {
+      const bool truth = rbit(&am[2u * g], att[c]);
+      bbs_u32x4 b = bbs_philox_block(seed, 0u, 0u, g * n_cells + c,
+                                     BBS_PURPOSE_DATA_NOISE);
+      int state;
+      if (!truth)
+        state = bbs_u01f(b.x[0]) < prob_false_positive ? 1 : 0;
+      else
+        state = bbs_u01f(b.x[0]) < prob_false_negative ? 0 : 1;
+      if (bbs_u01f(b.x[1]) < prob_missing) state = 3;
+      matrix[(size_t)g * n_cells + c] = (int8_t)state;
+    }
+  }
+  if (true_parent) std::memcpy(true_parent, parent, n * sizeof(uint32_t));
+  if (attachment) std::memcpy(attachment, att.data(), n_cells * sizeof(uint32_t));
+  return BBS_OK;
+}
+
+int bbs_random_trees(uint64_t seed, uint32_t n_genes, uint32_t n_trees,
+                     uint64_t *dm_rows) {
+  if (!dm_rows) return fail(nullptr, BBS_ERR_ARG, "dm_rows is NULL");
+  if (n_genes < 2u || n_genes > BBS_MAX_GENES)
+    return fail(nullptr, BBS_ERR_SIZE, "unsupported gene count %u", n_genes);
+  const uint32_t n = n_genes + 1u;
+  for (uint32_t t = 0; t < n_trees; t++) {
+    uint32_t parent[BBS_MAX_NODES];
+    gen_tree_parent(seed, 0u, t, BBS_PURPOSE_TREE_SAMPLE, n, parent);
+    int rc = parent_to_rows(parent, n, nullptr, dm_rows + (size_t)t * n * 2u);
+    if (rc) return rc;
+  }
+  return BBS_OK;
+}
+
+int bbs_initial_tree(uint64_t seed, uint32_t job, uint32_t chain,
+                     uint32_t n_genes, uint64_t *am, uint64_t *dm) {
+  if (n_genes < 2u || n_genes > BBS_MAX_GENES)
+    return fail(nullptr, BBS_ERR_SIZE, "unsupported gene count %u", n_genes);
+  uint32_t parent[BBS_MAX_NODES];
+  gen_tree_parent(seed, job, chain, BBS_PURPOSE_INIT_TREE, n_genes + 1u, parent);
+  return parent_to_rows(parent, n_genes + 1u, am, dm);
+}
+
+int bbs_parent_vector_to_matrix(const uint32_t *parent, uint32_t n_nodes,
+                                uint64_t *am, uint64_t *dm) {
+  if (!parent) return fail(nullptr, BBS_ERR_ARG, "parent is NULL");
+  int rc = parent_to_rows(parent, n_nodes, am, dm);
+  if (rc) return fail(nullptr, rc, "not a rooted tree over %u nodes", n_nodes);
+  return BBS_OK;
+}
+
+int bbs_pruefer_to_parent_vector(const uint32_t *code, uint32_t code_len,
+                                 uint32_t *parent) {
+  if (!code || !parent) return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  int rc = pruefer_to_parent(code, code_len, parent);
+  if (rc) return fail(nullptr, rc, "invalid Pruefer code");
+  return BBS_OK;
+}
+
+int64_t bbs_tree_to_graphviz(const uint64_t *am, const uint64_t *dm,
+                             uint32_t n_nodes, char *buf, uint64_t cap) {
+  (void)am;
+  if (!dm || n_nodes < 1u || n_nodes > BBS_MAX_NODES) return BBS_ERR_ARG;
+  TextOut out{buf, cap, 0};
+  out.put("digraph G {\n");
+  out.put("node [color=deeppink4, style=filled, fontcolor=white];\n");
+  for (uint32_t i = 0; i + 1u < n_nodes; i++)
+    out.put(std::to_string(rows_parent(dm, n_nodes, i)) + " -> " +
+            std::to_string(i) + ";\n");
+  out.put("}\n");
+  out.finish();
+  return (int64_t)out.len;
+}
+
+int64_t bbs_tree_to_newick(const uint64_t *am, const uint64_t *dm,
+                           uint32_t n_nodes, char *buf, uint64_t cap) {
+  (void)am;
+  if (!dm || n_nodes < 1u || n_nodes > BBS_MAX_NODES) return BBS_ERR_ARG;
+  std::vector<std::vector<uint32_t>> children(n_nodes);
+  for (uint32_t i = 0; i + 1u < n_nodes; i++)
+    children[rows_parent(dm, n_nodes, i)].push_back(i);
+  TextOut out{buf, cap, 0};
+  newick_rec(children, n_nodes - 1u, out, 0u);
+  out.put("\n");
+  out.finish();
+  return (int64_t)out.len;
+}
+
+} /* extern "C" */

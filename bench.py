@@ -1,0 +1,195 @@
+"""Benchmark of the MI355X bbSCITE chain search (one GPU).
+
+Workload (the plain method at the 96-bit limit): 94 genes x 95 cells,
+synthetic data from seed 2 (uniform Pruefer tree, uniform attachment,
+-ad 0.3, -fd 1e-4, 15 % missing), 1,024 chains with beta-change moves
+(beta prior mean 0.3, sd 0.05, -e 0.1 with the default move mix), one
+wavefront per chain. One bench step = every chain advances by
+--chain-steps steps (default 100,000); the default warmup 1 + steps 9 runs
+exactly 1,024 chains x 1,000,000 steps.
+
+Printed: one JSON line with the chain steps/s of the timed steps, the
+roofline fraction of the chain kernel against the gfx950 int32 VALU peak
+(HIP-event kernel time on the launch stream), the bulk-scoring throughput
+(2^20 random trees against a 94 x 95 matrix) and the CPU oracle timed on
+the host cores of the same machine.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from bbscite_amd import _capi as B  # noqa: E402
+
+G, C = 94, 95
+N = G + 1
+W32 = (G + 31) // 32
+# Algorithmic integer work per scored tree / chain step: per (cell, node)
+# pair 2 AND + 2 popcount per 32-bit gene word, plus 2 subtractions.
+ALG_INT_OPS = N * C * (4 * W32 + 2)  # 126,350 at 94 x 95
+# gfx950 int32 VALU peak: 256 CUs x 64 lanes x 2.4 GHz.
+P_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
+FPGA_KSTEPS = 2938.25  # reference's published chain steps/s at 95 x 94 (kilo)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_context(n_chains, n_steps):
+    m, _, _ = B.generate_dataset(2, G, C, 1e-4, 0.3, 0.15)
+    mut, kno = B.encode_matrix(m)
+    p = B.default_params(
+        n_chains=n_chains, n_steps=n_steps, seed=2, alpha_mean=1e-4, beta_mean=0.3,
+        beta_sd=0.05, prob_beta_change=0.1,
+    )
+    return p, mut, kno
+
+
+def cpu_baseline(p, mut, kno, seconds_budget=20.0):
+    """The CPU oracle's chain loop on the host cores (bounded sample)."""
+    try:
+        from tests import _oracle as O
+    except Exception as e:  # pragma: no cover
+        log("cpu baseline unavailable:", e)
+        return None
+    cores = min(16, len(os.sched_getaffinity(0)))
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    # calibrate on one core, then size the sample to ~seconds_budget of CPU work
+    r1, _ = O.bench_chains(p, s, 1, 200, 1)
+    steps_total = max(cores * 200, int(r1 * seconds_budget))
+    n_chains = cores * 4
+    n_steps = max(100, steps_total // n_chains)
+    rate, _ = O.bench_chains(p, s, n_chains, n_steps, cores)
+    return {
+        "value": rate,
+        "unit": "chain steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle chain loop at 94x95, {n_chains} chains x {n_steps} steps "
+                  f"(one chain per thread; 1-core rate {r1:.0f} steps/s)",
+        "one_core_value": r1,
+    }
+
+
+def scoring_bench(reps=5):
+    """2^20 uniform random trees (seed 3) against one 94 x 95 iid matrix."""
+    rng = np.random.default_rng(3)
+    u = rng.random((G, C))
+    m = np.zeros((G, C), np.int8)
+    m[u < 0.3] = 1
+    m[(u >= 0.3) & (u < 0.4)] = 3
+    mut, kno = B.encode_matrix(m)
+    p = B.default_params(n_chains=1, n_steps=1)
+    n_trees = 1 << 20
+    with B.Context(p, mut, kno, G) as ctx:
+        ms, _ = ctx.bench_score_random(3, n_trees, reps, with_prior=False, n_head=0)
+    k_ms = float(np.median(ms))
+    trees_s = n_trees / (k_ms / 1e3)
+    achieved = ALG_INT_OPS * trees_s / 1e12
+    return {
+        "trees_per_s": trees_s,
+        "kernel_ms": k_ms,
+        "roofline": {"bound": "valu-int32", "achieved": achieved, "peak": P_INT32_TOPS,
+                     "unit": "TOP/s", "frac": achieved / P_INT32_TOPS},
+        "counted_bits_per_s": trees_s * 4 * N ** 3,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chain-steps", type=int, default=100_000)
+    ap.add_argument("--chains", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-score", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(chains shard across GPUs with one final argmax exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    S = a.chain_steps
+    total = (a.warmup + a.steps) * S
+    p, mut, kno = make_context(a.chains, total)
+    log(f"[bench] 94x95, {a.chains} chains, {a.warmup}+{a.steps} windows of {S} steps")
+    ctx = B.Context(p, mut, kno, G)
+    ctx.begin()
+    for i in range(a.warmup):
+        ms = ctx.advance(S)
+        log(f"[bench] warmup window {i}: {ms:.1f} ms")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ms = ctx.advance(S)  # blocks until the window's kernel has finished
+        kernel_ms.append(ms)
+        log(f"[bench] window {i}: {ms:.1f} ms")
+    wall = time.perf_counter() - t0
+    steps_done = a.steps * S * a.chains
+    value = steps_done / wall
+    k_rate = steps_done / (sum(kernel_ms) / 1e3)
+    achieved = ALG_INT_OPS * k_rate / 1e12
+    _, _, best_beta, best_score = ctx.best()
+    ctx.close()
+
+    score = None if a.no_score else scoring_bench()
+    if score:
+        log(f"[bench] scoring: {score['trees_per_s']:.3e} trees/s")
+    cpu = None if a.no_cpu else cpu_baseline(p, mut, kno)
+
+    out = {
+        "metric": "chain steps/s at 94x95, batched persistent vs plain launch; p99 job latency",
+        "value": value,
+        "unit": "chain steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / (FPGA_KSTEPS * 1e3),
+        "baseline_ref": "bbSCITE96 on Stratix 10, 2,938.25 ksteps/s at 95x94 (paper Table 2)",
+        "dtype": "u32+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain chain kernel, 94 genes x 95 cells, 1024 chains, "
+                        f"{a.warmup + a.steps} x {S} steps per chain",
+            "chains": a.chains,
+            "chain_steps_per_bench_step": S,
+            "genes": G,
+            "cells": C,
+        },
+        "roofline": {
+            "bound": "valu-int32",
+            "achieved": achieved,
+            "peak": P_INT32_TOPS,
+            "unit": "TOP/s",
+            "frac": achieved / P_INT32_TOPS,
+            "traffic": None,
+            "alg_int_ops_per_step": ALG_INT_OPS,
+            "kernel": "chain_kernel<2,2,3>",
+        },
+        "kernel_steps_per_s": k_rate,
+        "best_log_likelihood": best_score,
+        "best_beta": best_beta,
+        "scoring": score,
+        "batched": None,
+        "serving_p99_ms": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,239 @@
+/*
+ * bbscite: C-ABI of the MI355X implementation of bbSCITE's MCMC
+ * mutation-tree search.
+ *
+ * This is the drop-in boundary for the chain loop of the reference. Each
+ * entry point names the reference interface it replaces. All pointers are
+ * host memory owned by the caller; the library copies in and out and owns
+ * all device memory. No exception crosses this boundary: every function
+ * returns a status (0 = ok, < 0 = error) and bbs_last_error() describes the
+ * last failure.
+ *
+ * Bit rows: a tree row or a data row is two uint64_t words, bit i of the row
+ * being bit (i % 64) of word (i / 64). A tree of n_nodes = n_genes + 1 nodes
+ * is n_nodes rows; node n_nodes - 1 is the root.
+ *   am[x] bit y : x is an ancestor-or-self of y   ("ancestor" matrix, the
+ *                 descendants of x; src/MutationTree.hpp:645-650)
+ *   dm[x] bit y : y is an ancestor-or-self of x   ("descendant" matrix, the
+ *                 ancestors of x; src/MutationTree.hpp:718-723)
+ *   is_mutated[c] / is_known[c] bit g : gene g of cell c
+ *                 (src/main.cpp:103-144)
+ */
+#ifndef BBSCITE_H
+#define BBSCITE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBS_MAX_GENES 127u
+#define BBS_MAX_CELLS 128u
+#define BBS_MAX_NODES 128u
+
+/* Status codes. */
+#define BBS_OK 0
+#define BBS_ERR_ARG (-1)    /* invalid argument                         */
+#define BBS_ERR_SIZE (-2)   /* genes/cells beyond the supported limits  */
+#define BBS_ERR_HIP (-3)    /* HIP runtime error                        */
+#define BBS_ERR_BUDGET (-4) /* request exceeds the device memory budget */
+#define BBS_ERR_STATE (-5)  /* call out of order (e.g. results before run) */
+#define BBS_ERR_NODEV (-6)  /* no usable gfx950 device                  */
+
+/* Move types (src/MoveType.hpp:23-42). */
+#define BBS_MOVE_CHANGE_BETA 0u
+#define BBS_MOVE_PRUNE_REATTACH 1u
+#define BBS_MOVE_SWAP_NODES 2u
+#define BBS_MOVE_SWAP_SUBTREES 3u
+
+/*
+ * Run parameters; the counterpart of class Parameters
+ * (src/Parameters.hpp:39-45 defaults, :61-281 flags). Move probabilities are
+ * given unnormalised, as on the command line, and normalised by their sum in
+ * float as Parameters::get_prob_* do (src/Parameters.hpp:451-541,609-612).
+ */
+typedef struct bbs_params {
+  float alpha_mean;            /* -fd : false-positive rate, 6.04e-5        */
+  float beta_mean;             /* -ad : false-negative rate, 0.4309          */
+  float beta_sd;               /* -sd : prior sd of beta, 0.1                */
+  float gamma;                 /* -g  : likelihood exponent, 1               */
+  float prob_beta_change;      /* -e  : 0                                    */
+  float prob_prune_n_reattach; /* -move_probs p q r : 0.55 0.4 0.05          */
+  float prob_swap_nodes;
+  float prob_swap_subtrees;
+  float beta_jump_scaling_chi; /* -x  : 10; beta jump sd = sd / chi          */
+  uint32_t n_chains;           /* -r                                         */
+  uint32_t n_steps;            /* -l                                         */
+  uint64_t seed;               /* -seed (the reference ignores it)           */
+} bbs_params;
+
+/* Fill *p with the reference defaults (src/Parameters.hpp:39-45),
+ * n_chains = n_steps = 0 and seed = 0. */
+void bbs_params_default(bbs_params *p);
+
+typedef struct bbs_ctx bbs_ctx;
+
+/* One step of one chain, for replay against the CPU oracle. The fields are
+ * ModificationParameters (src/MutationTree.hpp:84-89) plus the proposal's
+ * score and the Metropolis outcome (src/Application.hpp:441-456). */
+typedef struct bbs_step_trace {
+  uint32_t move_type;
+  uint32_t v, w, parent_of_v, parent_of_w, descendant_of_v, nondescendant_of_v;
+  float new_beta;
+  float proposed_score;
+  uint32_t flags; /* bit 0: accepted; bit 1: new chain-local maximum */
+} bbs_step_trace;
+
+/*
+ * Replaces Application's constructor (src/Application.hpp:88-186) minus the
+ * host-side move pre-generation: binds the encoded data and parameters to
+ * the first gfx950 device. Initial chain states are drawn from the seed
+ * (uniform Pruefer code per chain, src/Application.hpp:149-164) unless
+ * bbs_set_initial_states is called. The chain count is used as given (the
+ * reference's emulation build, pipeline capacity 1, src/main.cpp:28-29).
+ * is_mutated / is_known: n_cells rows each.
+ */
+int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
+               const uint64_t *is_mutated, const uint64_t *is_known,
+               bbs_ctx **out);
+
+/* Replaces the initial-state generation of src/Application.hpp:149-164 with
+ * caller-provided states: am/dm [n_chains][n_nodes] rows, beta[n_chains].
+ * score may be NULL, in which case the device scores the initial trees. */
+int bbs_set_initial_states(bbs_ctx *ctx, const uint64_t *am, const uint64_t *dm,
+                           const float *beta, const float *score);
+
+/* Record a bbs_step_trace for every step of chains [0, n_traced_chains). */
+int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
+
+/* Replaces Application::run_simulation (src/Application.hpp:197-219): runs
+ * all chains for n_steps steps and blocks; *makespan_ms (nullable) receives
+ * the device time in milliseconds, measured with HIP events. */
+int bbs_run(bbs_ctx *ctx, float *makespan_ms);
+
+/* Replaces get_best_am/get_best_dm/get_best_beta/get_best_score
+ * (src/Application.hpp:224-256): the first strictly greatest proposed tree
+ * in chain-major step order (src/Application.hpp:456). am/dm: n_nodes rows
+ * each; any pointer may be NULL. */
+int bbs_get_best(bbs_ctx *ctx, uint64_t *am, uint64_t *dm, float *beta,
+                 float *score);
+
+/* Split form of bbs_run for callers that run the chains in windows:
+ * bbs_begin draws (or keeps) the initial states and clears the maxima;
+ * bbs_advance runs n more steps of every chain (steps continue where the
+ * previous window ended, so any split gives the same result as one
+ * bbs_run). No reference counterpart: the reference runs all steps in one
+ * kernel invocation. */
+int bbs_begin(bbs_ctx *ctx);
+int bbs_advance(bbs_ctx *ctx, uint32_t n_steps, float *makespan_ms);
+
+/* Per-chain outcome after bbs_run (no reference counterpart; the reference
+ * keeps only the global best). Each array has n_chains entries and may be
+ * NULL. best_step is UINT32_MAX for a chain without a finite maximum. */
+int bbs_get_chain_results(bbs_ctx *ctx, float *best_score, uint32_t *best_step,
+                          float *final_score, float *final_beta);
+
+/* Final current state of every chain: am/dm [n_chains][n_nodes] rows. */
+int bbs_get_chain_states(bbs_ctx *ctx, uint64_t *am, uint64_t *dm);
+
+/* Copy the recorded trace: [n_traced_chains][n_steps] records. */
+int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out);
+
+#define BBS_SCORE_NO_PRIOR 1u
+
+/*
+ * Replaces TreeScorer::logscore_tree (src/TreeScorer.hpp:131-210) over many
+ * trees: dm_rows [n_trees][n_nodes] rows (ancestor sets, the rows the scorer
+ * reads, src/TreeScorer.hpp:155), betas [n_trees] or NULL for beta_mean.
+ * ll[n_trees] receives the log-likelihood (plus the beta prior unless
+ * flags has BBS_SCORE_NO_PRIOR). counts (nullable) receives, per tree and
+ * cell, the four counts n00,n01,n10,n11 (observed x true) at the cell's
+ * best attachment node, the first maximal node: [n_trees][n_cells][4].
+ */
+int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
+                    const float *betas, uint32_t flags, float *ll,
+                    uint8_t *counts);
+
+/*
+ * Bulk scoring with device-resident inputs (the config of
+ * cpu_scoring_benchmark/main.cpp:168-212: random trees scored against one
+ * matrix): n_trees uniform random trees are drawn on the device from seed
+ * (bbs_random_trees gives the same trees on the host), then scored reps
+ * times. kernel_ms[reps] receives each scoring launch's device time (HIP
+ * events on the launch stream); ll_head receives the first n_head scores.
+ */
+int bbs_bench_score_random(bbs_ctx *ctx, uint64_t seed, uint32_t n_trees,
+                           uint32_t reps, uint32_t flags, float *kernel_ms,
+                           float *ll_head, uint32_t n_head);
+
+/* Random trees drawn on the device, copied to the host: am (nullable), dm
+ * [n_trees][n_nodes] rows. purpose 2 gives the initial chain states,
+ * purpose 6 the bulk-scoring trees. */
+int bbs_random_trees_device(bbs_ctx *ctx, uint64_t seed, uint32_t purpose,
+                            uint32_t n_trees, uint64_t *am, uint64_t *dm);
+
+/* Device values of expf(x), logf(x) and (float)log(1.0 - x) as the chain
+ * kernel computes them (the libm calls of src/Application.hpp:454 and
+ * src/TreeScorer.hpp:124,136-137), for comparison with the host libm. */
+int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
+                  float *log_out, float *log1m_out);
+
+/* Message of the last failure on ctx, or of the last failed bbs_create on
+ * this thread when ctx is NULL. */
+const char *bbs_last_error(const bbs_ctx *ctx);
+
+void bbs_destroy(bbs_ctx *ctx);
+
+/* ---- host-only helpers (no device needed) ------------------------------ */
+
+/* Parse the reference CSV format (src/main.cpp:52-101): entries 0..3
+ * separated by spaces, genes as rows. matrix receives n_genes x n_cells
+ * int8 entries (capacity BBS_MAX_GENES x BBS_MAX_CELLS is always enough
+ * for accepted inputs). */
+int bbs_parse_csv(const char *text, uint64_t length, int8_t *matrix,
+                  uint32_t *n_genes, uint32_t *n_cells);
+
+/* Encode a genes x cells matrix into per-cell rows (src/main.cpp:103-144):
+ * 0 -> known, not mutated; 1, 2 -> known, mutated; 3 -> unknown. */
+int bbs_encode_matrix(const int8_t *matrix, uint32_t n_genes, uint32_t n_cells,
+                      uint64_t *is_mutated, uint64_t *is_known);
+
+/* Synthetic data: uniform Pruefer tree over n_genes + 1 nodes, uniform cell
+ * attachment, truth = ancestor-or-self, then 0->1 with prob
+ * prob_false_positive, 1->0 with prob prob_false_negative, then missing (3)
+ * with prob prob_missing. matrix [n_genes][n_cells]; true_parent
+ * [n_genes + 1] and attachment [n_cells] are optional. */
+int bbs_generate_dataset(uint64_t seed, uint32_t n_genes, uint32_t n_cells,
+                         float prob_false_positive, float prob_false_negative,
+                         float prob_missing, int8_t *matrix,
+                         uint32_t *true_parent, uint32_t *attachment);
+
+/* Uniform random trees (Pruefer codes) as dm rows [n_trees][n_genes + 1]. */
+int bbs_random_trees(uint64_t seed, uint32_t n_genes, uint32_t n_trees,
+                     uint64_t *dm_rows);
+
+/* The initial tree of (job, chain) as the device draws it (am/dm rows). */
+int bbs_initial_tree(uint64_t seed, uint32_t job, uint32_t chain,
+                     uint32_t n_genes, uint64_t *am, uint64_t *dm);
+
+/* Parent vector -> am/dm rows (src/MutationTree.hpp:379-409); Pruefer code
+ * -> parent vector (src/MutationTree.hpp:427-476). */
+int bbs_parent_vector_to_matrix(const uint32_t *parent, uint32_t n_nodes,
+                                uint64_t *am, uint64_t *dm);
+int bbs_pruefer_to_parent_vector(const uint32_t *code, uint32_t code_len,
+                                 uint32_t *parent);
+
+/* Graphviz and Newick text of a tree given by its am/dm rows
+ * (src/MutationTree.hpp:799-837,880-894). Returns the text length, or a
+ * negative status; writes at most cap bytes including the terminating 0. */
+int64_t bbs_tree_to_graphviz(const uint64_t *am, const uint64_t *dm,
+                             uint32_t n_nodes, char *buf, uint64_t cap);
+int64_t bbs_tree_to_newick(const uint64_t *am, const uint64_t *dm,
+                           uint32_t n_nodes, char *buf, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBSCITE_H */

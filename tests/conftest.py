@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+    sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    # Build only what is missing: the oracle (gcc) and the HIP library (hipcc).
+    from tests import _oracle  # noqa: F401
+
+    if not os.path.exists(_oracle.LIB_PATH):
+        _oracle.build()
+    lib = os.path.join(ROOT, "bbscite_amd", "libbbscite.so")
+    if not os.path.exists(lib):
+        from bbscite_amd import build
+
+        build.build_lib()

@@ -1,0 +1,266 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bar: bit-exact for every integer/bit result (counts, moves, trees, flags)
+and for the float scores too (the reference's float order is reproduced, so
+the 1e-4 tolerance of the specification is met with zero difference).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from bbscite_amd import _capi as B
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dataset(seed, G, C, ad=0.2, fd=1e-4, missing=0.1):
+    m, par, att = B.generate_dataset(seed, G, C, fd, ad, missing)
+    mut, kno = B.encode_matrix(m)
+    return m, mut, kno
+
+
+def _iid_matrix(seed, G, C, p1=0.3, p3=0.1):
+    rng = np.random.default_rng(seed)
+    u = rng.random((G, C))
+    m = np.zeros((G, C), np.int8)
+    m[u < p1] = 1
+    m[(u >= p1) & (u < p1 + p3)] = 3
+    return m
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+SHAPES = [(2, 1), (4, 6), (20, 17), (31, 64), (32, 65), (63, 64), (64, 65), (94, 95),
+          (95, 96), (127, 128)]
+
+
+@pytest.mark.parametrize("G,C", SHAPES)
+def test_score_trees_bit_exact(G, C):
+    say(f"[score] G={G} C={C}")
+    m = _iid_matrix(G * 1000 + C, G, C)
+    if C > 2:
+        m[:, 1] = 3  # a cell without any known entry
+    mut, kno = B.encode_matrix(m)
+    n_trees = 64
+    dm = B.random_trees(7 + G, G, n_trees)
+    rng = np.random.default_rng(G)
+    betas = rng.uniform(0.05, 0.95, n_trees).astype(np.float32)
+    p = B.default_params(n_chains=1, n_steps=1)
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    with B.Context(p, mut, kno, G) as ctx:
+        ll, cnt = ctx.score_trees(dm, with_prior=False, counts=True)
+        llb = ctx.score_trees(dm, betas=betas, with_prior=True)
+    for t in range(n_trees):
+        tree = O.tree_from_rows(np.zeros_like(dm[t]), dm[t], p.beta_mean)  # scorer reads dm
+        T, ocnt, _ = O.logscore_tree(s, tree, with_prior=False, counts=True)
+        assert _bits(ll[t]) == _bits(T), (t, ll[t], T)
+        assert np.array_equal(cnt[t], ocnt), t
+        tree.beta = float(betas[t])
+        Tb = O.logscore_tree(s, tree, with_prior=True)
+        assert _bits(llb[t]) == _bits(Tb), (t, llb[t], Tb)
+
+
+@pytest.mark.parametrize("G", [2, 16, 63, 64, 94, 127])
+def test_device_random_trees_match_host(G):
+    say(f"[trees] G={G}")
+    m = _iid_matrix(3, G, 8)
+    mut, kno = B.encode_matrix(m)
+    p = B.default_params(n_chains=4, n_steps=1, seed=99)
+    with B.Context(p, mut, kno, G) as ctx:
+        dm_dev = ctx.random_trees_device(5, B.PURPOSE_TREE_SAMPLE, 40)
+        am_i, dm_i = ctx.random_trees_device(99, B.PURPOSE_INIT_TREE, 4, with_am=True)
+    assert np.array_equal(dm_dev, B.random_trees(5, G, 40))
+    for c in range(4):
+        am_h, dm_h = B.initial_tree(99, 0, c, G)
+        assert np.array_equal(am_i[c], am_h) and np.array_equal(dm_i[c], dm_h)
+        t = O.Tree()
+        O.lib().orc_initial_tree(p, 0, c, G, t)
+        assert np.array_equal(O.rows(t, "am"), am_h)
+        assert np.array_equal(O.rows(t, "dm"), dm_h)
+
+
+def test_device_libm_matches_glibc():
+    say("[libm]")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.uniform(0, 1, 400000).astype(np.float32),
+        rng.uniform(-110, 90, 400000).astype(np.float32),
+        rng.integers(0, 2**32, 400000, dtype=np.uint64).astype(np.uint32).view(np.float32),
+        np.array([0.0, 1.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, 88.72, -103.97], np.float32),
+    ])
+    m = _iid_matrix(1, 4, 4)
+    mut, kno = B.encode_matrix(m)
+    with B.Context(B.default_params(n_chains=1, n_steps=1), mut, kno, 4) as ctx:
+        e, l, l1 = ctx.eval_libm(x)
+    with np.errstate(all="ignore"):
+        ref_e = np.exp(x.astype(np.float32))
+    # numpy's float32 exp/log are not glibc's; compare with the oracle's
+    # glibc through ctypes on a subset, and exactly through libm.so.6 below.
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    libm.expf.restype = ctypes.c_float
+    libm.expf.argtypes = [ctypes.c_float]
+    libm.logf.restype = ctypes.c_float
+    libm.logf.argtypes = [ctypes.c_float]
+    libm.log.restype = ctypes.c_double
+    libm.log.argtypes = [ctypes.c_double]
+    sub = np.arange(0, len(x), 7)
+    for i in sub:
+        xi = float(x[i])
+        ge = np.float32(libm.expf(xi))
+        gl = np.float32(libm.logf(xi))
+        gl1 = np.float32(libm.log(1.0 - xi)) if not math.isnan(xi) else np.float32(np.nan)
+        for got, ref in ((e[i], ge), (l[i], gl), (l1[i], gl1)):
+            if math.isnan(ref):
+                assert math.isnan(got), (hex(x[i:i+1].view(np.uint32)[0]), got, ref)
+            else:
+                assert _bits(got) == _bits(ref), (hex(x[i:i+1].view(np.uint32)[0]), got, ref)
+    del ref_e
+
+
+def _chain_case(G, C, seed, n_chains, n_steps, p_beta, beta_mean=0.25, beta_sd=0.1,
+                ad=0.2, fd=1e-4, missing=0.1, alpha=None):
+    m, mut, kno = _dataset(seed, G, C, ad=ad, fd=fd, missing=missing)
+    p = B.default_params(
+        n_chains=n_chains, n_steps=n_steps, seed=seed,
+        alpha_mean=alpha if alpha is not None else max(fd, 1e-6),
+        beta_mean=beta_mean, beta_sd=beta_sd, prob_beta_change=p_beta,
+    )
+    return p, mut, kno
+
+
+CHAIN_CASES = [
+    # G, C, seed, chains, steps, p_beta, beta_mean, beta_sd
+    (4, 15, 1, 3, 400, 0.0, 0.25, 0.1),
+    (16, 20, 2, 4, 300, 0.2, 0.25, 0.1),
+    (63, 64, 3, 4, 200, 0.1, 0.2, 0.1),
+    (64, 33, 4, 2, 150, 0.1, 0.25, 0.1),
+    (94, 95, 5, 3, 150, 0.1, 0.3, 0.05),
+    (127, 128, 6, 2, 100, 0.1, 0.4309, 0.1),
+]
+
+
+@pytest.mark.parametrize("case", CHAIN_CASES, ids=lambda c: f"G{c[0]}C{c[1]}")
+def test_chain_trace_replay(case):
+    G, C, seed, R, S, pb, bm, bsd = case
+    say(f"[chain] G={G} C={C} chains={R} steps={S}")
+    p, mut, kno = _chain_case(G, C, seed, R, S, pb, beta_mean=bm, beta_sd=bsd)
+    with B.Context(p, mut, kno, G) as ctx:
+        ctx.set_trace(R)
+        ctx.run()
+        trace = ctx.trace(R)
+        bs, st, fs, fb = ctx.chain_results()
+        am_f, dm_f = ctx.chain_states()
+        best_am, best_dm, best_beta, best_score = ctx.best()
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    for c in range(R):
+        res, otr = O.run_chain(p, s, c, c == 0, trace=True)
+        for field in ("move_type", "v", "w", "parent_of_v", "parent_of_w",
+                      "descendant_of_v", "nondescendant_of_v", "flags"):
+            bad = np.nonzero(trace[c][field] != otr[field])[0]
+            assert len(bad) == 0, (c, field, bad[:5], trace[c][bad[:3]], otr[bad[:3]])
+        for field in ("new_beta", "proposed_score"):
+            bad = np.nonzero(_bits(trace[c][field]) != _bits(otr[field]))[0]
+            assert len(bad) == 0, (c, field, bad[:5], trace[c][field][bad[:3]], otr[field][bad[:3]])
+        assert _bits(fs[c]) == _bits(res.final_score)
+        assert np.array_equal(am_f[c], O.rows(res.final_tree, "am"))
+        assert np.array_equal(dm_f[c], O.rows(res.final_tree, "dm"))
+        assert st[c] == res.best_step
+        if res.best_step != 0xFFFFFFFF:
+            assert _bits(bs[c]) == _bits(res.best_score)
+    t, sc, ch, stp = O.run(p, s)
+    assert _bits(best_score) == _bits(sc)
+    assert np.array_equal(best_am, O.rows(t, "am"))
+    assert np.array_equal(best_dm, O.rows(t, "dm"))
+    assert _bits(best_beta) == _bits(t.beta)
+    # every move type occurs
+    types = set(np.unique(trace["move_type"]).tolist())
+    assert {1, 2, 3} <= types and (pb == 0 or 0 in types)
+
+
+def test_advance_windows_equal_one_run():
+    say("[windows]")
+    p, mut, kno = _chain_case(30, 40, 9, 8, 300, 0.1)
+    with B.Context(p, mut, kno, 30) as ctx:
+        ctx.run()
+        one = ctx.chain_results(), ctx.chain_states(), ctx.best()
+    with B.Context(p, mut, kno, 30) as ctx:
+        ctx.begin()
+        for n in (1, 99, 100, 100):
+            ctx.advance(n)
+        split = ctx.chain_results(), ctx.chain_states(), ctx.best()
+    for a, b in zip(one[0], split[0]):
+        assert np.array_equal(a, b)
+    for a, b in zip(one[1], split[1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(one[2][0], split[2][0]) and one[2][3] == split[2][3]
+
+
+def test_caller_initial_states():
+    """bbs_set_initial_states with and without scores: same as the oracle
+    started from the same trees."""
+    say("[init states]")
+    G, C, R = 12, 10, 3
+    p, mut, kno = _chain_case(G, C, 4, R, 120, 0.1)
+    trees = [B.parent_vector_to_matrix(B.pruefer_to_parent_vector(
+        np.random.default_rng(c).integers(0, G + 1, G - 1))) for c in range(R)]
+    am = np.stack([t[0] for t in trees])
+    dm = np.stack([t[1] for t in trees])
+    beta = np.full(R, 0.3, np.float32)
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    for give_score in (False, True):
+        with B.Context(p, mut, kno, G) as ctx:
+            score = None
+            if give_score:
+                score = np.array([O.logscore_tree(s, O.tree_from_rows(am[c], dm[c], 0.3))
+                                  for c in range(R)], np.float32)
+            ctx.set_initial_states(am, dm, beta, score)
+            ctx.run()
+            bs, st, fs, fb = ctx.chain_results()
+        for c in range(R):
+            res = O.run_chain(p, s, c, c == 0, init=O.tree_from_rows(am[c], dm[c], 0.3))
+            assert _bits(fs[c]) == _bits(res.final_score)
+            assert st[c] == res.best_step
+
+
+def test_bench_score_random_head_matches_oracle():
+    say("[bench score head]")
+    G, C = 94, 95
+    m = _iid_matrix(3, G, C)
+    mut, kno = B.encode_matrix(m)
+    p = B.default_params(n_chains=1, n_steps=1)
+    with B.Context(p, mut, kno, G) as ctx:
+        ms, head = ctx.bench_score_random(3, 4096, 2, with_prior=False, n_head=32)
+    assert (ms > 0).all()
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    dm = B.random_trees(3, G, 32)
+    rate, ll = O.bench_score(s, dm, p.beta_mean, 1)
+    assert np.array_equal(_bits(head), _bits(ll))
+
+
+def test_errors_are_reported():
+    m = _iid_matrix(1, 4, 4)
+    mut, kno = B.encode_matrix(m)
+    with pytest.raises(B.BbsError) as e:
+        B.Context(B.default_params(n_chains=1, n_steps=1, beta_mean=1.5), mut, kno, 4)
+    assert e.value.code == B.ERR_ARG
+    with B.Context(B.default_params(n_chains=2, n_steps=10), mut, kno, 4) as ctx:
+        with pytest.raises(B.BbsError) as e:
+            ctx.best()
+        assert e.value.code == B.ERR_STATE
+        with pytest.raises(B.BbsError):
+            ctx.set_trace(5)
+        ctx.begin()
+        with pytest.raises(B.BbsError):
+            ctx.advance(11)
