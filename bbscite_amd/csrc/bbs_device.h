@@ -133,6 +133,23 @@ __device__ __forceinline__ uint32_t parent_of(const uint64_t (&dm)[NS][NS],
   return result;
 }
 
+/* popcount(x) + acc as one v_bcnt_u32_b32 (the compiler would otherwise
+ * re-associate a chain of popcounts into separate counts plus adds). */
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
+/* IEEE maximum of two arithmetic results as one v_max_f32: both operands
+ * are quiet (never signalling) NaNs at worst, so no canonicalisation is
+ * needed; a NaN operand yields the other operand, as fmaxf. */
+__device__ __forceinline__ float vmax(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 /* Mutation data of the cells this lane scores. */
 template <int CS, int W>
 struct Cells {

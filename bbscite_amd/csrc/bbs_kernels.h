@@ -61,7 +61,10 @@ struct TreeGenArgs {
 
 void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w);
 hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream);
-hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream);
+/* cells32: [128][10] words for the lane-per-tree kernel (used when counts
+ * are not requested), or null to use the lane-per-cell kernel. */
+hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
+                        const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
 hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,
                        hipStream_t stream);

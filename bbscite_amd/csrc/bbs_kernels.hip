@@ -384,6 +384,110 @@ __global__ __launch_bounds__(64) void score_kernel(ScoreArgs a) {
   }
 }
 
+/* ------------------------------------------- bulk score, lane = tree */
+
+/*
+ * Bulk TreeScorer over many trees with one tree per lane: the 64 lanes of a
+ * wave score 64 different trees against the same data, so every lane does
+ * useful work whatever the cell count. The cell data are wave-uniform
+ * (scalar loads, SGPR operands); each lane keeps its per-cell running
+ * maxima in VGPRs (CMAX of them) and streams its tree's ancestor rows, one
+ * node per iteration. Cells beyond n_cells carry no known entry: their
+ * maximum is -0.0 and adding it leaves the ordered cell sum unchanged.
+ * cells32: [CMAX][10] words: K&M words 0..3, K&~M words 0..3, |K&~M|,
+ * |K&M| (as floats).
+ */
+template <int CMAX, int W>
+__global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
+                                                          const uint32_t *__restrict__ cells32) {
+  /* per cell: (K&M w0..w2, |K&~M|) (K&~M w0..w2, |K&M|) for W <= 3, and a
+   * third quad (K&M w3, K&~M w3, 0, 0) for W = 4 */
+  constexpr int Q = W <= 3 ? 2 : 3;
+  __shared__ uint4 cl[CMAX * Q];
+  for (uint32_t c = threadIdx.x; c < (uint32_t)CMAX; c += blockDim.x) {
+    const uint32_t *g = cells32 + 10u * c;
+    cl[Q * c] = make_uint4(g[0], g[1], g[2], g[8]);
+    cl[Q * c + 1] = make_uint4(g[4], g[5], g[6], g[9]);
+    if (Q == 3) cl[Q * c + 2] = make_uint4(g[3], g[7], 0u, 0u);
+  }
+  __syncthreads();
+  const uint32_t N = a.n_nodes;
+  const uint32_t lanes_total = gridDim.x * blockDim.x;
+  for (uint32_t tree0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+       tree0 < a.n_trees; tree0 += lanes_total) {
+    const uint32_t tree = tree0 + lane_id();
+    const bool live = tree < a.n_trees;
+    const uint32_t t = live ? tree : a.n_trees - 1u;
+    LogProbs L;
+    L.L00 = a.L00;
+    L.L10 = a.L10;
+    float prior;
+    if (a.betas) {
+      const float beta = a.betas[t];
+      L.L01 = bbs_logf(beta);
+      L.L11 = bbs_logf_of_double(1.0 - (double)beta);
+      prior = logscore_beta(a.prior, beta);
+    } else {
+      L.L01 = a.L01;
+      L.L11 = a.L11;
+      prior = a.prior_value;
+    }
+    const uint4 *rows = reinterpret_cast<const uint4 *>(a.dm) + (uint64_t)t * N;
+    float best[CMAX];
+    /* one (cell, node) pair: the reference's four masked popcounts */
+    auto pair = [&](const uint4 *cp, const uint4 &A) {
+      const uint4 q0 = cp[0], q1 = cp[1];
+      uint32_t n11 = bcnt_acc(q0.x & A.x, 0u);
+      uint32_t n01 = bcnt_acc(q1.x & A.x, 0u);
+      if (W > 1) {
+        n11 = bcnt_acc(q0.y & A.y, n11);
+        n01 = bcnt_acc(q1.y & A.y, n01);
+      }
+      if (W > 2) {
+        n11 = bcnt_acc(q0.z & A.z, n11);
+        n01 = bcnt_acc(q1.z & A.z, n01);
+      }
+      if (W > 3) {
+        const uint4 q2 = cp[2];
+        n11 = bcnt_acc(q2.x & A.w, n11);
+        n01 = bcnt_acc(q2.y & A.w, n01);
+      }
+      return pair_score(n01, n11, __uint_as_float(q0.w), __uint_as_float(q1.w), L);
+    };
+    uint4 A = rows[0];
+#pragma unroll
+    for (int c = 0; c < CMAX; c++) best[c] = pair(cl + Q * c, A);
+    for (uint32_t v = 1; v < N; v++) {
+      A = rows[v];
+      /* an opaque base keeps the cell reads inside the node loop: hoisting
+       * all of them would need CMAX * 4 Q registers */
+      uint32_t base = 0u;
+      asm volatile("" : "+v"(base));
+      const uint4 *cb = reinterpret_cast<const uint4 *>(
+          reinterpret_cast<const char *>(cl) + base);
+#pragma unroll
+      for (int c = 0; c < CMAX; c++) best[c] = vmax(best[c], pair(cb + Q * c, A));
+    }
+    /* A NaN at node 0 sticks in the reference's maximum; it needs an
+     * infinite log-probability, i.e. beta (or alpha) exactly 0 or 1. */
+    const bool inf_l = !(fabsf(L.L01) <= 3.4e38f) || !(fabsf(L.L11) <= 3.4e38f) ||
+                       !(fabsf(L.L00) <= 3.4e38f) || !(fabsf(L.L10) <= 3.4e38f);
+    if (inf_l) {
+      const uint4 A0 = rows[0];
+#pragma unroll
+      for (int c = 0; c < CMAX; c++) {
+        const float s0 = pair(cl + Q * c, A0);
+        if (s0 != s0) best[c] = s0;
+      }
+    }
+    float T = 0.0f;
+#pragma unroll
+    for (int c = 0; c < CMAX; c++)
+      if ((uint32_t)c < a.n_cells) T = T + best[c];
+    if (live) a.ll[tree] = a.with_prior ? T + prior : T;
+  }
+}
+
 /* ------------------------------------------------------- random trees */
 
 /* Uniform random trees from Pruefer codes drawn from the Philox stream
@@ -555,9 +659,26 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream) {
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream) {
+hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
+                        const uint32_t *cells32) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w);
+  if (!counts && cells32 != nullptr) {
+    const int cmax = a.n_cells <= 32u ? 32 : a.n_cells <= 64u ? 64 : a.n_cells <= 96u ? 96 : 128;
+    const int wq = (int)((a.n_nodes - 1u + 31u) / 32u);
+    uint32_t groups = (a.n_trees + 255u) / 256u;
+    if (groups > 16384u) groups = 16384u;
+    if (groups == 0u) return hipSuccess;
+#define XL(CM_, W_)                                                            \
+  if (cmax == CM_ && wq == W_) {                                               \
+    hipLaunchKernelGGL((score_lanes_kernel<CM_, W_>), dim3(groups), dim3(256), \
+                       0, stream, a, cells32);                                 \
+    return hipGetLastError();                                                  \
+  }
+    XL(32, 1) XL(32, 2) XL(32, 3) XL(32, 4) XL(64, 1) XL(64, 2) XL(64, 3) XL(64, 4)
+    XL(96, 1) XL(96, 2) XL(96, 3) XL(96, 4) XL(128, 1) XL(128, 2) XL(128, 3) XL(128, 4)
+#undef XL
+  }
   uint32_t blocks = a.n_trees < 65536u ? a.n_trees : 65536u;
   if (blocks == 0u) return hipSuccess;
   dim3 grid(blocks), block(64);

@@ -48,8 +48,8 @@ struct bbs_ctx {
   int device = -1;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf cells, am, dm, beta, score, best_am, best_dm, best_beta, best_score,
-      best_step, trace;
+  DevBuf cells, cells32, am, dm, beta, score, best_am, best_dm, best_beta,
+      best_score, best_step, trace;
   uint32_t n_traced = 0;
   bool states_set = false;   /* initial states present on the device */
   bool score_known = false;  /* initial scores present on the device  */
@@ -336,6 +336,23 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
     }
   }
 
+  /* the same data as [128][10] words for the lane-per-tree scorer */
+  std::vector<uint32_t> cells32(128u * 10u, 0u);
+  for (uint32_t c = 0; c < n_cells; c++) {
+    uint32_t a = 0, b = 0;
+    for (int w = 0; w < 4; w++) {
+      uint32_t km = (uint32_t)(cells[4u * c + (w >> 1)] >> (32 * (w & 1)));
+      uint32_t knm = (uint32_t)(cells[4u * c + 2u + (w >> 1)] >> (32 * (w & 1)));
+      cells32[10u * c + w] = km;
+      cells32[10u * c + 4u + w] = knm;
+      b += (uint32_t)__builtin_popcount(km);
+      a += (uint32_t)__builtin_popcount(knm);
+    }
+    float fa = (float)a, fb = (float)b;
+    std::memcpy(&cells32[10u * c + 8u], &fa, 4);
+    std::memcpy(&cells32[10u * c + 9u], &fb, 4);
+  }
+
   int rc;
   hipError_t e = hipSetDevice(dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -349,6 +366,7 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
   const size_t rows = (size_t)p.n_chains * ctx->n_nodes * 2u * sizeof(uint64_t);
   const size_t per_chain = (size_t)p.n_chains * sizeof(float);
   if ((rc = dev_alloc(ctx, ctx->cells, cells.size() * sizeof(uint64_t))) ||
+      (rc = dev_alloc(ctx, ctx->cells32, cells32.size() * sizeof(uint32_t))) ||
       (rc = dev_alloc(ctx, ctx->am, rows)) || (rc = dev_alloc(ctx, ctx->dm, rows)) ||
       (rc = dev_alloc(ctx, ctx->best_am, rows)) ||
       (rc = dev_alloc(ctx, ctx->best_dm, rows)) ||
@@ -363,6 +381,9 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
   }
   e = hipMemcpy(ctx->cells.p, cells.data(), cells.size() * sizeof(uint64_t),
                 hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(ctx->cells32.p, cells32.data(), cells32.size() * sizeof(uint32_t),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     g_create_error = std::string("hipMemcpy failed: ") + hipGetErrorString(e);
     bbs_destroy(ctx);
@@ -376,7 +397,7 @@ void bbs_destroy(bbs_ctx *ctx) {
   if (!ctx) return;
   if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&ctx->cells,     &ctx->am,         &ctx->dm,
+  DevBuf *bufs[] = {&ctx->cells,     &ctx->cells32,    &ctx->am,         &ctx->dm,
                     &ctx->beta,      &ctx->score,      &ctx->best_am,
                     &ctx->best_dm,   &ctx->best_beta,  &ctx->best_score,
                     &ctx->best_step, &ctx->trace};
@@ -632,7 +653,10 @@ int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
   hipError_t e = hipMemcpy(d_rows.p, dm_rows, rows, hipMemcpyHostToDevice);
   if (e == hipSuccess && betas)
     e = hipMemcpy(d_betas.p, betas, (size_t)n_trees * 4u, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = bbs::launch_score(s, counts != nullptr, ctx->stream);
+  if (e == hipSuccess)
+    e = bbs::launch_score(s, counts != nullptr, ctx->stream,
+                          (flags & BBS_SCORE_CELL_LANES) ? nullptr
+                                                         : (const uint32_t *)ctx->cells32.p);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpy(ll, d_ll.p, (size_t)n_trees * 4u, hipMemcpyDeviceToHost);
@@ -681,7 +705,10 @@ int bbs_bench_score_random(bbs_ctx *ctx, uint64_t seed, uint32_t n_trees,
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   for (uint32_t r = 0; r < reps && e == hipSuccess; r++) {
     e = hipEventRecord(ctx->ev0, ctx->stream);
-    if (e == hipSuccess) e = bbs::launch_score(s, false, ctx->stream);
+    if (e == hipSuccess)
+      e = bbs::launch_score(s, false, ctx->stream,
+                            (flags & BBS_SCORE_CELL_LANES) ? nullptr
+                                                           : (const uint32_t *)ctx->cells32.p);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     float ms = 0.f;

@@ -113,7 +113,20 @@ def main():
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+
+        n = a.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {a.copy_calibration} MiB")
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
               "(chains shard across GPUs with one final argmax exchange; see DESIGN.md)",

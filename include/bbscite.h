@@ -141,6 +141,9 @@ int bbs_get_chain_states(bbs_ctx *ctx, uint64_t *am, uint64_t *dm);
 int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out);
 
 #define BBS_SCORE_NO_PRIOR 1u
+/* Use the lane-per-cell scorer (the chain kernel's) instead of the
+ * lane-per-tree bulk scorer; results are identical. */
+#define BBS_SCORE_CELL_LANES 2u
 
 /*
  * Replaces TreeScorer::logscore_tree (src/TreeScorer.hpp:131-210) over many

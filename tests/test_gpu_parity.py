@@ -40,8 +40,8 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-SHAPES = [(2, 1), (4, 6), (20, 17), (31, 64), (32, 65), (63, 64), (64, 65), (94, 95),
-          (95, 96), (127, 128)]
+SHAPES = [(2, 1), (4, 6), (20, 17), (31, 64), (32, 65), (33, 97), (63, 64), (64, 65),
+          (94, 95), (95, 96), (96, 33), (127, 128)]
 
 
 @pytest.mark.parametrize("G,C", SHAPES)
@@ -58,8 +58,12 @@ def test_score_trees_bit_exact(G, C):
     p = B.default_params(n_chains=1, n_steps=1)
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     with B.Context(p, mut, kno, G) as ctx:
-        ll, cnt = ctx.score_trees(dm, with_prior=False, counts=True)
+        ll, cnt = ctx.score_trees(dm, with_prior=False, counts=True)  # lane per cell
+        ll_lanes = ctx.score_trees(dm, with_prior=False)  # lane per tree
         llb = ctx.score_trees(dm, betas=betas, with_prior=True)
+        llb_cells = ctx.score_trees(dm, betas=betas, with_prior=True, cell_lanes=True)
+    assert np.array_equal(_bits(ll), _bits(ll_lanes))
+    assert np.array_equal(_bits(llb), _bits(llb_cells))
     for t in range(n_trees):
         tree = O.tree_from_rows(np.zeros_like(dm[t]), dm[t], p.beta_mean)  # scorer reads dm
         T, ocnt, _ = O.logscore_tree(s, tree, with_prior=False, counts=True)
@@ -234,6 +238,29 @@ def test_caller_initial_states():
             assert st[c] == res.best_step
 
 
+def test_score_trees_degenerate_beta():
+    """beta exactly 0 or 1 gives infinite log-probabilities; NaN pairs at
+    node 0 stick in the maximum as in the reference (both scorers)."""
+    G, C = 10, 12
+    m = _iid_matrix(5, G, C)
+    mut, kno = B.encode_matrix(m)
+    dm = B.random_trees(2, G, 70)
+    betas = np.array([0.0, 1.0] * 35, np.float32)
+    p = B.default_params(n_chains=1, n_steps=1)
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    with B.Context(p, mut, kno, G) as ctx:
+        a = ctx.score_trees(dm, betas=betas, with_prior=False)
+        b = ctx.score_trees(dm, betas=betas, with_prior=False, cell_lanes=True)
+    for t in range(70):
+        tree = O.tree_from_rows(np.zeros_like(dm[t]), dm[t], float(betas[t]))
+        ref = O.logscore_tree(s, tree, with_prior=False)
+        for got in (a[t], b[t]):
+            if math.isnan(ref):
+                assert math.isnan(got), t
+            else:
+                assert _bits(got) == _bits(ref), (t, got, ref)
+
+
 def test_bench_score_random_head_matches_oracle():
     say("[bench score head]")
     G, C = 94, 95
@@ -242,7 +269,10 @@ def test_bench_score_random_head_matches_oracle():
     p = B.default_params(n_chains=1, n_steps=1)
     with B.Context(p, mut, kno, G) as ctx:
         ms, head = ctx.bench_score_random(3, 4096, 2, with_prior=False, n_head=32)
+        ms2, head2 = ctx.bench_score_random(3, 4096, 1, with_prior=False, n_head=32,
+                                            cell_lanes=True)
     assert (ms > 0).all()
+    assert np.array_equal(_bits(head), _bits(head2))
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     dm = B.random_trees(3, G, 32)
     rate, ll = O.bench_score(s, dm, p.beta_mean, 1)
