@@ -87,6 +87,27 @@ TRACE_DTYPE = np.dtype(
     ]
 )
 
+class Job(C.Structure):
+    """bbs_job (include/bbscite.h): one dataset of a batched run."""
+
+    _fields_ = [
+        ("params", Params),
+        ("n_cells", C.c_uint32),
+        ("n_genes", C.c_uint32),
+        ("is_mutated", C.c_void_p),
+        ("is_known", C.c_void_p),
+        ("job_tag", C.c_uint32),
+    ]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("waves_per_group", C.c_uint32),
+        ("groups_per_cu", C.c_uint32),
+        ("order", C.c_uint32),
+    ]
+
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -108,7 +129,20 @@ _SIGS = {
     "bbs_bench_score_random": (C.c_int, [_P, _u64, _u32, _u32, _u32, _P, _P, _u32]),
     "bbs_random_trees_device": (C.c_int, [_P, _u64, _u32, _u32, _P, _P]),
     "bbs_eval_libm": (C.c_int, [_P, _P, _u32, _P, _P, _P]),
+    "bbs_measure_valu_peak": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bbs_last_error": (C.c_char_p, [_P]),
+    "bbs_batch_config_default": (None, [C.POINTER(BatchConfig)]),
+    "bbs_batch_create": (C.c_int, [C.POINTER(Job), _u32, C.POINTER(BatchConfig), C.POINTER(_P)]),
+    "bbs_batch_run": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "bbs_batch_total_steps": (_u64, [_P]),
+    "bbs_batch_get_best": (C.c_int, [_P, _u32, _P, _P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "bbs_batch_get_chain_results": (C.c_int, [_P, _u32, _P, _P, _P, _P]),
+    "bbs_batch_last_error": (C.c_char_p, [_P]),
+    "bbs_batch_destroy": (None, [_P]),
+    "bbs_batch_autotune": (
+        C.c_int,
+        [C.POINTER(Job), _u32, _u32, C.POINTER(BatchConfig), C.POINTER(C.c_float)],
+    ),
     "bbs_destroy": (None, [_P]),
     "bbs_parse_csv": (C.c_int, [C.c_char_p, _u64, _P, C.POINTER(_u32), C.POINTER(_u32)]),
     "bbs_encode_matrix": (C.c_int, [_P, _u32, _u32, _P, _P]),
@@ -375,6 +409,11 @@ class Context:
         )
         return ms, head
 
+    def measure_valu_peak(self):
+        ops, insts = C.c_double(), C.c_double()
+        self._c(lib().bbs_measure_valu_peak(self.h, C.byref(ops), C.byref(insts)))
+        return ops.value, insts.value
+
     def eval_libm(self, x):
         x = np.ascontiguousarray(x, np.float32)
         e = np.zeros_like(x)
@@ -388,3 +427,105 @@ class Context:
         am = np.zeros_like(dm) if with_am else None
         self._c(lib().bbs_random_trees_device(self.h, seed, purpose, n_trees, _ptr(am), _ptr(dm)))
         return (am, dm) if with_am else dm
+
+
+# ---------------------------------------------------------------- batches
+
+
+def _jobs_array(jobs):
+    """jobs: list of (params, is_mutated, is_known, n_genes[, job_tag])."""
+    arr = (Job * len(jobs))()
+    keep = []
+    for i, j in enumerate(jobs):
+        params, mut, kno, G = j[:4]
+        tag = j[4] if len(j) > 4 else 0
+        mut = np.ascontiguousarray(mut, np.uint64)
+        kno = np.ascontiguousarray(kno, np.uint64)
+        keep += [mut, kno]
+        arr[i].params = params
+        arr[i].n_cells = mut.shape[0]
+        arr[i].n_genes = G
+        arr[i].is_mutated = mut.ctypes.data
+        arr[i].is_known = kno.ctypes.data
+        arr[i].job_tag = tag
+    return arr, keep
+
+
+def _bcheck(rc, b=None):
+    if rc != OK:
+        msg = lib().bbs_batch_last_error(b)
+        raise BbsError(rc, msg.decode() if msg else "")
+
+
+def batch_config(waves_per_group=None, groups_per_cu=None, order=None):
+    c = BatchConfig()
+    lib().bbs_batch_config_default(C.byref(c))
+    if waves_per_group is not None:
+        c.waves_per_group = waves_per_group
+    if groups_per_cu is not None:
+        c.groups_per_cu = groups_per_cu
+    if order is not None:
+        c.order = order
+    return c
+
+
+def batch_autotune(jobs, probe_steps=2000):
+    arr, keep = _jobs_array(jobs)
+    cfg = BatchConfig()
+    ms = C.c_float()
+    _bcheck(lib().bbs_batch_autotune(arr, len(jobs), probe_steps, C.byref(cfg), C.byref(ms)))
+    return cfg, ms.value
+
+
+class Batch:
+    """Owns one bbs_batch: many datasets in one persistent, graph-captured launch."""
+
+    def __init__(self, jobs, cfg: BatchConfig | None = None):
+        arr, keep = _jobs_array(jobs)
+        self.jobs = jobs
+        h = C.c_void_p()
+        _bcheck(lib().bbs_batch_create(arr, len(jobs), C.byref(cfg) if cfg else None, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bbs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self):
+        ms = C.c_float()
+        _bcheck(lib().bbs_batch_run(self.h, C.byref(ms)), self.h)
+        return ms.value
+
+    @property
+    def total_steps(self):
+        return lib().bbs_batch_total_steps(self.h)
+
+    def chain_results(self, job):
+        R = self.jobs[job][0].n_chains
+        bs = np.zeros(R, np.float32)
+        st = np.zeros(R, np.uint32)
+        fs = np.zeros(R, np.float32)
+        fb = np.zeros(R, np.float32)
+        _bcheck(lib().bbs_batch_get_chain_results(self.h, job, _ptr(bs), _ptr(st), _ptr(fs), _ptr(fb)), self.h)
+        return bs, st, fs, fb
+
+    def best(self, job):
+        n = self.jobs[job][3] + 1
+        am = np.zeros((n, 2), np.uint64)
+        dm = np.zeros((n, 2), np.uint64)
+        b, s = C.c_float(), C.c_float()
+        _bcheck(lib().bbs_batch_get_best(self.h, job, _ptr(am), _ptr(dm), C.byref(b), C.byref(s)), self.h)
+        return am, dm, b.value, s.value

@@ -49,6 +49,15 @@ __device__ __forceinline__ uint64_t ballot(bool p) {
   return __ballot(p);
 }
 
+/* Order this wave's LDS writes before its later LDS reads (LDS executes a
+ * wave's operations in order; this keeps the compiler from moving them). The
+ * LDS scratch of score_tree belongs to one wave, so no workgroup barrier is
+ * needed and waves of one workgroup may run different chains. */
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 /* Number of set bits of m in lanes below this lane. */
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -230,8 +239,8 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
       uint32_t n11 = 0, n01 = 0;
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        n11 = (uint32_t)__popc(c.km[j][w] & A[w]) + n11;
-        n01 = (uint32_t)__popc(c.knm[j][w] & A[w]) + n01;
+        n11 = bcnt_acc(c.km[j][w] & A[w], n11);
+        n01 = bcnt_acc(c.knm[j][w] & A[w], n01);
       }
       float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
       best[j] = s;
@@ -249,8 +258,8 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
       uint32_t n11 = 0, n01 = 0;
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        n11 = (uint32_t)__popc(c.km[j][w] & A[w]) + n11;
-        n01 = (uint32_t)__popc(c.knm[j][w] & A[w]) + n01;
+        n11 = bcnt_acc(c.km[j][w] & A[w], n11);
+        n01 = bcnt_acc(c.knm[j][w] & A[w], n01);
       }
       float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
       if (COUNTS) {
@@ -260,7 +269,7 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
           arg11[j] = n11;
         }
       } else {
-        best[j] = fmaxf(best[j], s);
+        best[j] = vmax(best[j], s);
       }
     }
   };
@@ -291,7 +300,7 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
 #pragma unroll
   for (int j = 0; j < CS; j++)
     lds[lane_id() + 64u * (uint32_t)j] = nan0[j] ? __int_as_float(0x7fc00000) : best[j];
-  __syncthreads();
+  wave_lds_sync();
   float T = 0.0f;
   uint32_t cc = 0;
   for (; cc + 4u <= n_cells; cc += 4u) {
@@ -302,7 +311,7 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
     T = T + q.w;
   }
   for (; cc < n_cells; cc++) T = T + lds[cc];
-  __syncthreads();
+  wave_lds_sync();
   return rfl_f(T);
 }
 

@@ -59,6 +59,33 @@ struct TreeGenArgs {
   uint32_t n_trees, n_nodes;
 };
 
+/* One dataset of a batched run (device copy). Offsets index the pooled
+ * buffers of BatchArgs: cells in cells (4 words each), chains in the
+ * per-chain scalars, rows in the row buffers (2 words each). */
+struct BatchJob {
+  uint64_t seed;
+  uint64_t row_offset;
+  uint32_t job_tag, n_nodes, n_cells, shape;
+  uint32_t cell_offset, chain_offset, n_chains;
+  uint32_t step_begin, step_end, score_init;
+  float L00, L10, gamma, beta_mean;
+  PriorArgs prior;
+  bbs_move_probs probs;
+};
+
+/* items: (job << 16 | chain) in scheduling order; counter: work counter
+ * (reset to 0 by the init kernel). */
+struct BatchArgs {
+  const BatchJob *jobs;
+  const uint32_t *items;
+  uint32_t n_items;
+  uint32_t *counter;
+  const uint64_t *cells;
+  uint64_t *am, *dm, *best_am, *best_dm;
+  float *beta, *score, *best_beta, *best_score;
+  uint32_t *best_step;
+};
+
 void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w);
 hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream);
 /* cells32: [128][10] words for the lane-per-tree kernel (used when counts
@@ -66,6 +93,11 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream);
 hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
+hipError_t launch_batch_init(const BatchArgs &a, hipStream_t stream);
+hipError_t launch_batch(const BatchArgs &a, uint32_t groups, uint32_t waves_per_group,
+                        hipStream_t stream);
+hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
+                            uint32_t iters, hipStream_t stream);
 hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,
                        hipStream_t stream);
 

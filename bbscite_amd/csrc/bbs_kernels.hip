@@ -20,6 +20,16 @@
 
 namespace bbs {
 
+/* The register layouts instantiated: (NS node slots, CS cell slots, W gene
+ * words) for every gene/cell count up to 127 x 128. */
+#define BBS_SHAPES(X)                                                          \
+  X(1, 1, 1) X(1, 1, 2) X(1, 2, 1) X(1, 2, 2) X(2, 1, 2) X(2, 1, 3) X(2, 1, 4) \
+      X(2, 2, 2) X(2, 2, 3) X(2, 2, 4)
+
+#define BBS_SHAPE_CODE(NS_, CS_, W_) ((NS_)*100 + (CS_)*10 + (W_))
+
+__device__ void random_tree_wave(const TreeGenArgs &a, uint32_t tree);
+
 /* ------------------------------------------------------------------ moves */
 
 /* The proposal of one move for the rows of this lane
@@ -130,32 +140,49 @@ __device__ __forceinline__ void apply_move(const MoveCtx<NS> &m,
   }
 }
 
-/* ------------------------------------------------------------ chain kernel */
+/* ------------------------------------------------------------ chain body */
 
+/* Everything one wavefront needs to advance one chain; all pointers point
+ * at this chain's own rows / scalars. */
+struct ChainView {
+  const uint64_t *cells;
+  uint64_t *am, *dm, *best_am, *best_dm; /* n_nodes rows each             */
+  float *beta, *score, *best_beta, *best_score;
+  uint32_t *best_step;
+  bbs_step_trace *trace; /* [step] or null                        */
+  uint64_t seed;
+  uint32_t job, chain; /* Philox counter words                          */
+  uint32_t step_begin, step_end;
+  uint32_t n_cells, n_nodes;
+  bool is_first_chain; /* its step 0 seeds the maximum (i == 0)        */
+  bool score_init;
+  float L00, L10, gamma;
+  PriorArgs prior;
+  bbs_move_probs probs;
+};
+
+/* Steps [step_begin, step_end) of one chain: the WorkKernel loop body of
+ * src/Application.hpp:412-487 with the state resident in VGPRs. */
 template <int NS, int CS, int W, bool TRACE>
-__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
-  __shared__ __align__(16) float lds[64 * CS];
+__device__ __forceinline__ void run_chain(const ChainView &a, float *lds) {
   const uint32_t lane = lane_id();
-  const uint32_t chain = blockIdx.x;
-  if (chain >= a.n_chains) return;
   const uint32_t N = a.n_nodes;
 
   Cells<CS, W> cells;
   load_cells<CS, W>(a.cells, a.n_cells, cells);
 
   uint64_t am[NS][NS], dm[NS][NS], bam[NS][NS], bdm[NS][NS];
-  const uint64_t row_base = (uint64_t)chain * N;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     const uint32_t x = lane + 64u * (uint32_t)k;
 #pragma unroll
     for (int wd = 0; wd < NS; wd++) {
-      am[k][wd] = x < N ? a.am[2u * (row_base + x) + (uint32_t)wd] : 0ull;
-      dm[k][wd] = x < N ? a.dm[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+      am[k][wd] = x < N ? a.am[2u * x + (uint32_t)wd] : 0ull;
+      dm[k][wd] = x < N ? a.dm[2u * x + (uint32_t)wd] : 0ull;
     }
   }
 
-  float cur_beta = a.beta[chain];
+  float cur_beta = *a.beta;
   LogProbs L;
   L.L00 = a.L00;
   L.L10 = a.L10;
@@ -169,25 +196,24 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
                                              unused01, unused11) +
                 cur_prior;
   else
-    cur_score = a.score[chain];
+    cur_score = *a.score;
 
-  float best_score = a.best_score[chain];
-  float best_beta = a.best_beta[chain];
-  uint32_t best_step = a.best_step[chain];
+  float best_score = *a.best_score;
+  float best_beta = *a.best_beta;
+  uint32_t best_step = *a.best_step;
   const bool have_best = best_step != 0xffffffffu;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     const uint32_t x = lane + 64u * (uint32_t)k;
 #pragma unroll
     for (int wd = 0; wd < NS; wd++) {
-      bam[k][wd] = (have_best && x < N) ? a.best_am[2u * (row_base + x) + (uint32_t)wd] : 0ull;
-      bdm[k][wd] = (have_best && x < N) ? a.best_dm[2u * (row_base + x) + (uint32_t)wd] : 0ull;
+      bam[k][wd] = (have_best && x < N) ? a.best_am[2u * x + (uint32_t)wd] : 0ull;
+      bdm[k][wd] = (have_best && x < N) ? a.best_dm[2u * x + (uint32_t)wd] : 0ull;
     }
   }
 
   for (uint32_t step = a.step_begin; step < a.step_end; step++) {
-    const bbs_raw_move raw =
-        bbs_draw_move(a.seed, a.job, a.chain_offset + chain, step, N, a.probs);
+    const bbs_raw_move raw = bbs_draw_move(a.seed, a.job, a.chain, step, N, a.probs);
     MoveCtx<NS> m;
     m.type = raw.move_type;
     uint32_t v = raw.raw_v, w = raw.raw_w;
@@ -257,10 +283,9 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
       corr = (float)nd_v / (float)nd_w;
     const float acc_prob = corr * bbs_expf((prop_score - cur_score) * a.gamma);
     const bool accept = acc_prob > raw.acceptance_level;
-    const bool new_max = (a.chain_offset + chain == 0u && step == 0u) ||
-                         prop_score > best_score;
+    const bool new_max = (a.is_first_chain && step == 0u) || prop_score > best_score;
 
-    if (TRACE && chain < a.n_traced && lane == 0u) {
+    if (TRACE && a.trace != nullptr && lane == 0u) {
       bbs_step_trace tr;
       tr.move_type = m.type;
       tr.v = v;
@@ -272,7 +297,7 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
       tr.new_beta = nb;
       tr.proposed_score = prop_score;
       tr.flags = (accept ? 1u : 0u) | (new_max ? 2u : 0u);
-      a.trace[(uint64_t)chain * a.trace_stride + step] = tr;
+      a.trace[step] = tr;
     }
 
     if (new_max) {
@@ -308,26 +333,155 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
     if (x < N) {
 #pragma unroll
       for (int wd = 0; wd < NS; wd++) {
-        a.am[2u * (row_base + x) + (uint32_t)wd] = am[k][wd];
-        a.dm[2u * (row_base + x) + (uint32_t)wd] = dm[k][wd];
-        a.best_am[2u * (row_base + x) + (uint32_t)wd] = bam[k][wd];
-        a.best_dm[2u * (row_base + x) + (uint32_t)wd] = bdm[k][wd];
+        a.am[2u * x + (uint32_t)wd] = am[k][wd];
+        a.dm[2u * x + (uint32_t)wd] = dm[k][wd];
+        a.best_am[2u * x + (uint32_t)wd] = bam[k][wd];
+        a.best_dm[2u * x + (uint32_t)wd] = bdm[k][wd];
       }
       if (NS == 1) {
-        a.am[2u * (row_base + x) + 1u] = 0ull;
-        a.dm[2u * (row_base + x) + 1u] = 0ull;
-        a.best_am[2u * (row_base + x) + 1u] = 0ull;
-        a.best_dm[2u * (row_base + x) + 1u] = 0ull;
+        a.am[2u * x + 1u] = 0ull;
+        a.dm[2u * x + 1u] = 0ull;
+        a.best_am[2u * x + 1u] = 0ull;
+        a.best_dm[2u * x + 1u] = 0ull;
       }
     }
   }
   if (lane == 0u) {
-    a.beta[chain] = cur_beta;
-    a.score[chain] = cur_score;
-    a.best_score[chain] = best_score;
-    a.best_beta[chain] = best_beta;
-    a.best_step[chain] = best_step;
+    *a.beta = cur_beta;
+    *a.score = cur_score;
+    *a.best_score = best_score;
+    *a.best_beta = best_beta;
+    *a.best_step = best_step;
   }
+}
+
+/* ------------------------------------------------------------ plain kernel */
+
+/* One dataset per launch, one wavefront per chain (the drop-in form). */
+template <int NS, int CS, int W, bool TRACE>
+__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
+  __shared__ __align__(16) float lds[64 * CS];
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
+  const uint64_t rows = 2ull * chain * a.n_nodes;
+  ChainView v;
+  v.cells = a.cells;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + chain;
+  v.score = a.score + chain;
+  v.best_beta = a.best_beta + chain;
+  v.best_score = a.best_score + chain;
+  v.best_step = a.best_step + chain;
+  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
+                                          : nullptr;
+  v.seed = a.seed;
+  v.job = a.job;
+  v.chain = a.chain_offset + chain;
+  v.step_begin = a.step_begin;
+  v.step_end = a.step_end;
+  v.n_cells = a.n_cells;
+  v.n_nodes = a.n_nodes;
+  v.is_first_chain = a.chain_offset + chain == 0u;
+  v.score_init = a.score_init != 0u;
+  v.L00 = a.L00;
+  v.L10 = a.L10;
+  v.gamma = a.gamma;
+  v.prior = a.prior;
+  v.probs = a.probs;
+  run_chain<NS, CS, W, TRACE>(v, lds);
+}
+
+/* ---------------------------------------------------------- batched kernel */
+
+/*
+ * Persistent form for many datasets at once: a fixed grid of wavefronts
+ * pulls (job, chain) items from a global counter (largest work first) and
+ * runs each chain for all of its steps; the shape of each job selects the
+ * register layout. Every wave leaves when the counter passes the last item.
+ */
+__global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
+  __shared__ __align__(16) float lds_all[4][128];
+  const uint32_t wave = threadIdx.x >> 6;
+  float *lds = lds_all[wave];
+  for (;;) {
+    uint32_t item = 0u;
+    if (lane_id() == 0u) item = atomicAdd(a.counter, 1u);
+    item = rfl(item);
+    if (item >= a.n_items) break;
+    const uint32_t packed = a.items[item];
+    const uint32_t j = packed >> 16, c = packed & 0xffffu;
+    const BatchJob &J = a.jobs[j];
+    const uint64_t rows = 2ull * ((uint64_t)J.row_offset + (uint64_t)c * J.n_nodes);
+    const uint32_t gc = J.chain_offset + c;
+    ChainView v;
+    v.cells = a.cells + 4ull * J.cell_offset;
+    v.am = a.am + rows;
+    v.dm = a.dm + rows;
+    v.best_am = a.best_am + rows;
+    v.best_dm = a.best_dm + rows;
+    v.beta = a.beta + gc;
+    v.score = a.score + gc;
+    v.best_beta = a.best_beta + gc;
+    v.best_score = a.best_score + gc;
+    v.best_step = a.best_step + gc;
+    v.trace = nullptr;
+    v.seed = J.seed;
+    v.job = J.job_tag;
+    v.chain = c;
+    v.step_begin = J.step_begin;
+    v.step_end = J.step_end;
+    v.n_cells = J.n_cells;
+    v.n_nodes = J.n_nodes;
+    v.is_first_chain = c == 0u;
+    v.score_init = J.score_init != 0u;
+    v.L00 = J.L00;
+    v.L10 = J.L10;
+    v.gamma = J.gamma;
+    v.prior = J.prior;
+    v.probs = J.probs;
+    switch (J.shape) {
+#define XS(NS_, CS_, W_)                                                       \
+  case BBS_SHAPE_CODE(NS_, CS_, W_):                                           \
+    run_chain<NS_, CS_, W_, false>(v, lds);                                    \
+    break;
+      BBS_SHAPES(XS)
+#undef XS
+    default:
+      break;
+    }
+  }
+}
+
+/* Initial states of every (job, chain) of a batch: uniform random trees
+ * from each job's own seed, as the plain form draws them. */
+__global__ __launch_bounds__(64) void batch_init_kernel(BatchArgs a) {
+  for (uint32_t item = blockIdx.x; item < a.n_items; item += gridDim.x) {
+    const uint32_t packed = a.items[item];
+    const uint32_t j = packed >> 16, c = packed & 0xffffu;
+    const BatchJob &J = a.jobs[j];
+    TreeGenArgs g;
+    const uint64_t rows = 2ull * ((uint64_t)J.row_offset + (uint64_t)c * J.n_nodes);
+    g.am = a.am + rows;
+    g.dm = a.dm + rows;
+    g.beta = a.beta + J.chain_offset + c;
+    g.beta_value = J.beta_mean;
+    g.seed = J.seed;
+    g.job = J.job_tag;
+    g.chain_offset = c;
+    g.purpose = BBS_PURPOSE_INIT_TREE;
+    g.n_trees = 1u;
+    g.n_nodes = J.n_nodes;
+    random_tree_wave(g, 0u);
+    if (lane_id() == 0u) {
+      a.best_step[J.chain_offset + c] = 0xffffffffu;
+      a.best_score[J.chain_offset + c] = __int_as_float(0xff800000);
+      a.best_beta[J.chain_offset + c] = __int_as_float(0xff800000);
+    }
+  }
+  if (blockIdx.x == 0u && lane_id() == 0u) *a.counter = 0u;
 }
 
 /* ------------------------------------------------------------ score kernel */
@@ -490,119 +644,153 @@ __global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
 
 /* ------------------------------------------------------- random trees */
 
-/* Uniform random trees from Pruefer codes drawn from the Philox stream
- * (purpose, chain = tree index + chain_offset); one wave per tree.
- * Writes am (nullable) and dm rows [tree][n_nodes][2] and beta (nullable). */
-__global__ __launch_bounds__(64) void random_trees_kernel(TreeGenArgs a) {
+/* One uniform random tree from a Pruefer code drawn from the Philox stream
+ * (chain word = chain_offset + tree), built by one wavefront: rows
+ * [tree][n_nodes][2] of am (nullable) and dm, and beta (nullable). */
+__device__ void random_tree_wave(const TreeGenArgs &a, uint32_t tree) {
   __shared__ uint32_t degree[128];
   __shared__ uint32_t parent[128];
   __shared__ uint32_t code[128];
   __shared__ uint64_t rows[128][2];
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
-  for (uint32_t tree = blockIdx.x; tree < a.n_trees; tree += gridDim.x) {
-    const uint32_t chain = a.chain_offset + tree;
-    /* code[i] ~ U{0..N-1}, i < N-2 (src/MutationTree.hpp:500-507) */
-    for (uint32_t i = lane; i + 2u < N; i += 64u) {
-      bbs_u32x4 b = bbs_philox_block(a.seed, a.job, chain, i >> 2, a.purpose);
-      code[i] = bbs_uint_below(b.x[i & 3u], N);
-    }
-    for (uint32_t i = lane; i < 128u; i += 64u) {
-      degree[i] = 1u;
-      parent[i] = N - 1u;
-    }
+  const uint32_t chain = a.chain_offset + tree;
+  /* code[i] ~ U{0..N-1}, i < N-2 (src/MutationTree.hpp:500-507) */
+  for (uint32_t i = lane; i + 2u < N; i += 64u) {
+    bbs_u32x4 b = bbs_philox_block(a.seed, a.job, chain, i >> 2, a.purpose);
+    code[i] = bbs_uint_below(b.x[i & 3u], N);
+  }
+  for (uint32_t i = lane; i < 128u; i += 64u) {
+    degree[i] = 1u;
+    parent[i] = N - 1u;
+  }
+  __syncthreads();
+  if (lane == 0u)
+    for (uint32_t i = 0; i + 2u < N; i++) degree[code[i]]++;
+  __syncthreads();
+  /* decode: repeatedly attach the smallest leaf (src/MutationTree.hpp:454-463) */
+  for (uint32_t i = 0; i + 2u < N; i++) {
+    bool leaf0 = lane < N && degree[lane] == 1u;
+    bool leaf1 = lane + 64u < N && degree[lane + 64u] == 1u;
+    uint64_t m0 = ballot(leaf0), m1 = ballot(leaf1);
+    uint32_t j = m0 ? (uint32_t)__ffsll((long long)m0) - 1u
+                    : 64u + (uint32_t)__ffsll((long long)m1) - 1u;
     __syncthreads();
-    if (lane == 0u)
-      for (uint32_t i = 0; i + 2u < N; i++) degree[code[i]]++;
-    __syncthreads();
-    /* decode: repeatedly attach the smallest leaf (src/MutationTree.hpp:454-463) */
-    for (uint32_t i = 0; i + 2u < N; i++) {
-      bool leaf0 = lane < N && degree[lane] == 1u;
-      bool leaf1 = lane + 64u < N && degree[lane + 64u] == 1u;
-      uint64_t m0 = ballot(leaf0), m1 = ballot(leaf1);
-      uint32_t j = m0 ? (uint32_t)__ffsll((long long)m0) - 1u
-                      : 64u + (uint32_t)__ffsll((long long)m1) - 1u;
-      __syncthreads();
-      if (lane == 0u) {
-        uint32_t c = code[i];
-        parent[j] = c;
-        degree[c]--;
-        degree[j]--;
-      }
-      __syncthreads();
+    if (lane == 0u) {
+      uint32_t c = code[i];
+      parent[j] = c;
+      degree[c]--;
+      degree[j]--;
     }
-    /* the remaining leaf other than the root hangs below the root: parent
-     * already initialised to N-1 (src/MutationTree.hpp:467-473) */
-    /* ancestor rows: dm[x] = {x} | dm[parent[x]], iterated to a fixed point */
-    uint64_t d[2][2];
-    for (int k = 0; k < 2; k++) {
-      const uint32_t x = lane + 64u * (uint32_t)k;
-      d[k][0] = (x < N && x < 64u) ? (1ull << x) : 0ull;
-      d[k][1] = (x < N && x >= 64u) ? (1ull << (x - 64u)) : 0ull;
-      if (x < N) {
-        /* root bit */
-        const uint32_t r = N - 1u;
-        d[k][r >> 6] |= 1ull << (r & 63u);
-      }
-    }
-    for (uint32_t iter = 0; iter < N; iter++) {
-      for (int k = 0; k < 2; k++) {
-        const uint32_t x = lane + 64u * (uint32_t)k;
-        rows[x][0] = d[k][0];
-        rows[x][1] = d[k][1];
-      }
-      __syncthreads();
-      bool changed = false;
-      for (int k = 0; k < 2; k++) {
-        const uint32_t x = lane + 64u * (uint32_t)k;
-        if (x < N && x != N - 1u) {
-          const uint32_t p = parent[x];
-          const uint64_t n0 = d[k][0] | rows[p][0], n1 = d[k][1] | rows[p][1];
-          changed = changed || n0 != d[k][0] || n1 != d[k][1];
-          d[k][0] = n0;
-          d[k][1] = n1;
-        }
-      }
-      __syncthreads();
-      if (ballot(changed) == 0ull) break;
-    }
-    const uint64_t base = (uint64_t)tree * N;
-    for (int k = 0; k < 2; k++) {
-      const uint32_t x = lane + 64u * (uint32_t)k;
-      if (x < N) {
-        a.dm[2u * (base + x)] = d[k][0];
-        a.dm[2u * (base + x) + 1u] = d[k][1];
-      }
-    }
-    if (a.am) {
-      /* am[x] bit y = dm[y] bit x: transpose through LDS */
-      for (int k = 0; k < 2; k++) {
-        const uint32_t x = lane + 64u * (uint32_t)k;
-        rows[x][0] = d[k][0];
-        rows[x][1] = d[k][1];
-      }
-      __syncthreads();
-      for (int k = 0; k < 2; k++) {
-        const uint32_t x = lane + 64u * (uint32_t)k;
-        uint64_t r0 = 0ull, r1 = 0ull;
-        if (x < N) {
-          for (uint32_t y = 0; y < N; y++) {
-            const bool bit = (rows[y][x >> 6] >> (x & 63u)) & 1u;
-            if (bit) {
-              if (y < 64u)
-                r0 |= 1ull << y;
-              else
-                r1 |= 1ull << (y - 64u);
-            }
-          }
-          a.am[2u * (base + x)] = r0;
-          a.am[2u * (base + x) + 1u] = r1;
-        }
-      }
-    }
-    if (a.beta && lane == 0u) a.beta[tree] = a.beta_value;
     __syncthreads();
   }
+  /* the remaining leaf other than the root hangs below the root: parent
+   * already initialised to N-1 (src/MutationTree.hpp:467-473) */
+  /* ancestor rows: dm[x] = {x} | dm[parent[x]], iterated to a fixed point */
+  uint64_t d[2][2];
+  for (int k = 0; k < 2; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+    d[k][0] = (x < N && x < 64u) ? (1ull << x) : 0ull;
+    d[k][1] = (x < N && x >= 64u) ? (1ull << (x - 64u)) : 0ull;
+    if (x < N) {
+      const uint32_t r = N - 1u;
+      d[k][r >> 6] |= 1ull << (r & 63u);
+    }
+  }
+  for (uint32_t iter = 0; iter < N; iter++) {
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      rows[x][0] = d[k][0];
+      rows[x][1] = d[k][1];
+    }
+    __syncthreads();
+    bool changed = false;
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      if (x < N && x != N - 1u) {
+        const uint32_t p = parent[x];
+        const uint64_t n0 = d[k][0] | rows[p][0], n1 = d[k][1] | rows[p][1];
+        changed = changed || n0 != d[k][0] || n1 != d[k][1];
+        d[k][0] = n0;
+        d[k][1] = n1;
+      }
+    }
+    __syncthreads();
+    if (ballot(changed) == 0ull) break;
+  }
+  const uint64_t base = (uint64_t)tree * N;
+  for (int k = 0; k < 2; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+    if (x < N) {
+      a.dm[2u * (base + x)] = d[k][0];
+      a.dm[2u * (base + x) + 1u] = d[k][1];
+    }
+  }
+  if (a.am) {
+    /* am[x] bit y = dm[y] bit x: transpose through LDS */
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      rows[x][0] = d[k][0];
+      rows[x][1] = d[k][1];
+    }
+    __syncthreads();
+    for (int k = 0; k < 2; k++) {
+      const uint32_t x = lane + 64u * (uint32_t)k;
+      uint64_t r0 = 0ull, r1 = 0ull;
+      if (x < N) {
+        for (uint32_t y = 0; y < N; y++) {
+          const bool bit = (rows[y][x >> 6] >> (x & 63u)) & 1u;
+          if (bit) {
+            if (y < 64u)
+              r0 |= 1ull << y;
+            else
+              r1 |= 1ull << (y - 64u);
+          }
+        }
+        a.am[2u * (base + x)] = r0;
+        a.am[2u * (base + x) + 1u] = r1;
+      }
+    }
+  }
+  if (a.beta && lane == 0u) a.beta[tree] = a.beta_value;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void random_trees_kernel(TreeGenArgs a) {
+  for (uint32_t tree = blockIdx.x; tree < a.n_trees; tree += gridDim.x)
+    random_tree_wave(a, tree);
+}
+
+/* ------------------------------------------------------ VALU peak probe */
+
+/* Integer VALU throughput calibration: 8 independent and/popcount chains per
+ * lane (2 int32 operations per step), long enough to hide all latency. */
+__global__ __launch_bounds__(256) void valu_peak_kernel(const uint32_t *in, uint32_t *out,
+                                                        uint32_t iters) {
+  const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+  uint32_t x[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) x[k] = in[(threadIdx.x * 8u + (uint32_t)k) & 1023u];
+  const uint32_t m = in[1024u + (threadIdx.x & 7u)];
+  for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      uint32_t t;
+      asm volatile("v_and_b32 %0, %1, %2" : "=v"(t) : "v"(x[k]), "v"(m));
+      asm volatile("v_bcnt_u32_b32 %0, %1, %2" : "=v"(x[k]) : "v"(t), "v"(x[k]));
+    }
+  }
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc += x[k];
+  out[tid] = acc;
+}
+
+hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
+                            uint32_t iters, hipStream_t stream) {
+  hipLaunchKernelGGL(valu_peak_kernel, dim3(blocks), dim3(256), 0, stream, in, out,
+                     iters);
+  return hipGetLastError();
 }
 
 /* ------------------------------------------------------------ libm probe */
@@ -627,10 +815,6 @@ hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1
 }
 
 /* ------------------------------------------------------------ dispatch */
-
-#define BBS_SHAPES(X)                                                          \
-  X(1, 1, 1) X(1, 1, 2) X(1, 2, 1) X(1, 2, 2) X(2, 1, 2) X(2, 1, 3) X(2, 1, 4) \
-      X(2, 2, 2) X(2, 2, 3) X(2, 2, 4)
 
 void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w) {
   *ns = (n_genes + 1u) <= 64u ? 1 : 2;
@@ -695,6 +879,22 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
   BBS_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_init(const BatchArgs &a, hipStream_t stream) {
+  uint32_t blocks = a.n_items < 65536u ? a.n_items : 65536u;
+  if (blocks == 0u) blocks = 1u;
+  hipLaunchKernelGGL(batch_init_kernel, dim3(blocks), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchArgs &a, uint32_t groups, uint32_t waves_per_group,
+                        hipStream_t stream) {
+  if (groups == 0u) groups = 1u;
+  if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_kernel, dim3(groups), dim3(64u * waves_per_group), 0, stream,
+                     a);
+  return hipGetLastError();
 }
 
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream) {

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -756,6 +757,48 @@ int bbs_random_trees_device(bbs_ctx *ctx, uint64_t seed, uint32_t purpose,
   return BBS_OK;
 }
 
+int bbs_measure_valu_peak(bbs_ctx *ctx, double *int32_ops_per_s,
+                          double *valu_insts_per_s) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16u;
+  const uint32_t iters = 4096u;
+  DevBuf din, dout;
+  int rc = dev_alloc(ctx, din, 2048u * 4u);
+  if (!rc) rc = dev_alloc(ctx, dout, (size_t)blocks * 256u * 4u);
+  if (rc) {
+    din.release();
+    dout.release();
+    return rc;
+  }
+  std::vector<uint32_t> h(2048);
+  for (uint32_t i = 0; i < 2048u; i++) h[i] = 0x9E3779B9u * (i + 1u);
+  hipError_t e = hipMemcpy(din.p, h.data(), 2048u * 4u, hipMemcpyHostToDevice);
+  float best = 1e30f;
+  for (int r = 0; r < 6 && e == hipSuccess; r++) {
+    e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess)
+      e = bbs::launch_valu_peak((const uint32_t *)din.p, (uint32_t *)dout.p, blocks,
+                                iters, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (r > 0 && ms < best) best = ms; /* first launch warms up */
+  }
+  din.release();
+  dout.release();
+  if (e != hipSuccess)
+    return fail(ctx, BBS_ERR_HIP, "bbs_measure_valu_peak: %s", hipGetErrorString(e));
+  const double lanes = (double)blocks * 256.0;
+  const double ops = lanes * iters * 16.0; /* 8 chains x (and + popcount) */
+  if (int32_ops_per_s) *int32_ops_per_s = ops / (best * 1e-3);
+  if (valu_insts_per_s) *valu_insts_per_s = ops / 64.0 / (best * 1e-3);
+  return BBS_OK;
+}
+
 int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
                   float *log_out, float *log1m_out) {
   if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
@@ -785,6 +828,411 @@ int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
   dm.release();
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, BBS_ERR_HIP, "bbs_eval_libm: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+/* ---------------------------------------------------------------- batch */
+
+} /* extern "C" */
+
+struct bbs_batch {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<bbs::BatchJob> jobs;
+  std::vector<uint32_t> items;
+  bbs_batch_config cfg{};
+  uint32_t n_cus = 0;
+  uint64_t total_steps = 0;
+  DevBuf d_jobs, d_items, d_counter, cells, am, dm, best_am, best_dm, beta, score,
+      best_beta, best_score, best_step;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ran = false;
+  std::string err;
+};
+
+namespace {
+
+int bfail(bbs_batch *b, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b)
+    b->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define BHIP_TRY(b, expr)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return bfail((b), BBS_ERR_HIP, "%s failed: %s", #expr,                   \
+                   hipGetErrorString(e_));                                     \
+  } while (0)
+
+int balloc(bbs_batch *b, DevBuf &d, size_t bytes) {
+  d.release();
+  if (bytes == 0) return BBS_OK;
+  hipError_t e = hipMalloc(&d.p, bytes);
+  if (e != hipSuccess) {
+    d.p = nullptr;
+    return bfail(b, e == hipErrorOutOfMemory ? BBS_ERR_BUDGET : BBS_ERR_HIP,
+                 "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  d.bytes = bytes;
+  return BBS_OK;
+}
+
+/* Scorer constants of one job, as the TreeScorer constructor computes them
+ * (src/TreeScorer.hpp:92-117), and its move thresholds. */
+int fill_job(const bbs_job &in, bbs::BatchJob &J, std::string &why) {
+  const bbs_params &p = in.params;
+  if (in.n_genes < 2u || in.n_genes > BBS_MAX_GENES || in.n_cells < 1u ||
+      in.n_cells > BBS_MAX_CELLS) {
+    why = "unsupported size";
+    return BBS_ERR_SIZE;
+  }
+  if (!(p.alpha_mean > 0.f && p.alpha_mean < 1.f) ||
+      !(p.beta_mean > 0.f && p.beta_mean < 1.f) || !(p.beta_sd > 0.f) ||
+      !(p.beta_jump_scaling_chi > 0.f) || p.prob_beta_change < 0.f ||
+      p.prob_prune_n_reattach < 0.f || p.prob_swap_nodes < 0.f ||
+      p.prob_swap_subtrees < 0.f ||
+      p.prob_beta_change + p.prob_prune_n_reattach + p.prob_swap_nodes +
+              p.prob_swap_subtrees ==
+          0.f ||
+      p.n_chains > 65535u || !in.is_mutated || !in.is_known) {
+    why = "invalid parameters";
+    return BBS_ERR_ARG;
+  }
+  J = bbs::BatchJob{};
+  J.seed = p.seed;
+  J.job_tag = in.job_tag;
+  J.n_nodes = in.n_genes + 1u;
+  J.n_cells = in.n_cells;
+  int ns, cs, w;
+  bbs::shape_of(in.n_genes, in.n_cells, &ns, &cs, &w);
+  J.shape = (uint32_t)(ns * 100 + cs * 10 + w);
+  J.n_chains = p.n_chains;
+  J.step_begin = 0u;
+  J.step_end = p.n_steps;
+  J.score_init = 1u;
+  J.L00 = (float)std::log(1.0 - (double)p.alpha_mean);
+  J.L10 = logf(p.alpha_mean);
+  J.gamma = p.gamma;
+  J.beta_mean = p.beta_mean;
+  double a = ((double)(1.0f - p.beta_mean) * std::pow((double)p.beta_mean, 2.0) /
+              std::pow((double)p.beta_sd, 2.0)) -
+             (double)p.beta_mean;
+  float bpa = (float)a, bpb = bpa * ((1.0f / p.beta_mean) - 1.0f);
+  J.prior.am1 = bpa - 1.0f;
+  J.prior.bm1 = bpb - 1.0f;
+  J.prior.lg_ab = lgamma_term(bpa + bpb);
+  J.prior.lg_a = lgamma_term(bpa);
+  J.prior.lg_b = lgamma_term(bpb);
+  float sum = p.prob_beta_change + p.prob_prune_n_reattach + p.prob_swap_nodes +
+              p.prob_swap_subtrees;
+  float pb = p.prob_beta_change / sum, pp = p.prob_prune_n_reattach / sum,
+        ps = p.prob_swap_nodes / sum;
+  J.probs.t_beta = pb;
+  J.probs.t_prune = pb + pp;
+  J.probs.t_swap = pb + pp + ps;
+  J.probs.beta_jump_sd = p.beta_sd / p.beta_jump_scaling_chi;
+  return BBS_OK;
+}
+
+/* Relative device cost of one step of a job (VALU work of the scorer). */
+double step_cost(const bbs::BatchJob &J) {
+  const int cs = (J.shape / 10) % 10, w = J.shape % 10;
+  return (double)J.n_nodes * cs * (4.0 * w + 14.0) + 600.0;
+}
+
+void order_items(bbs_batch *b) {
+  b->items.clear();
+  std::vector<uint32_t> order(b->jobs.size());
+  for (uint32_t j = 0; j < order.size(); j++) order[j] = j;
+  if (b->cfg.order == 0u) {
+    /* largest chain first: a greedy longest-processing-time schedule */
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+      const auto &A = b->jobs[x], &B = b->jobs[y];
+      return step_cost(A) * (A.step_end - A.step_begin) >
+             step_cost(B) * (B.step_end - B.step_begin);
+    });
+  }
+  for (uint32_t j : order)
+    for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
+}
+
+bbs::BatchArgs batch_args(bbs_batch *b) {
+  bbs::BatchArgs a{};
+  a.jobs = (const bbs::BatchJob *)b->d_jobs.p;
+  a.items = (const uint32_t *)b->d_items.p;
+  a.n_items = (uint32_t)b->items.size();
+  a.counter = (uint32_t *)b->d_counter.p;
+  a.cells = (const uint64_t *)b->cells.p;
+  a.am = (uint64_t *)b->am.p;
+  a.dm = (uint64_t *)b->dm.p;
+  a.best_am = (uint64_t *)b->best_am.p;
+  a.best_dm = (uint64_t *)b->best_dm.p;
+  a.beta = (float *)b->beta.p;
+  a.score = (float *)b->score.p;
+  a.best_beta = (float *)b->best_beta.p;
+  a.best_score = (float *)b->best_score.p;
+  a.best_step = (uint32_t *)b->best_step.p;
+  return a;
+}
+
+} // namespace
+
+extern "C" {
+
+void bbs_batch_config_default(bbs_batch_config *cfg) {
+  if (!cfg) return;
+  cfg->waves_per_group = 1u;
+  cfg->groups_per_cu = 8u;
+  cfg->order = 0u;
+}
+
+const char *bbs_batch_last_error(const bbs_batch *b) {
+  return b ? b->err.c_str() : g_create_error.c_str();
+}
+
+void bbs_batch_destroy(bbs_batch *b) {
+  if (!b) return;
+  if (b->device >= 0) (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  DevBuf *bufs[] = {&b->d_jobs,  &b->d_items,   &b->d_counter, &b->cells,
+                    &b->am,      &b->dm,        &b->best_am,   &b->best_dm,
+                    &b->beta,    &b->score,     &b->best_beta, &b->best_score,
+                    &b->best_step};
+  for (DevBuf *d : bufs) d->release();
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_config *cfg,
+                     bbs_batch **out) {
+  if (!out) return bfail(nullptr, BBS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!jobs || n_jobs == 0u) return bfail(nullptr, BBS_ERR_ARG, "no jobs");
+  if (n_jobs > 65535u) return bfail(nullptr, BBS_ERR_SIZE, "at most 65535 jobs per batch");
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
+    return bfail(nullptr, BBS_ERR_NODEV, "no HIP device available");
+  int dev = -1;
+  for (int d = 0; d < n_dev && dev < 0; d++)
+    if (is_gfx950(d)) dev = d;
+  if (dev < 0) return bfail(nullptr, BBS_ERR_NODEV, "no gfx950 device found");
+
+  bbs_batch *b = new bbs_batch();
+  b->device = dev;
+  if (cfg)
+    b->cfg = *cfg;
+  else
+    bbs_batch_config_default(&b->cfg);
+  if (b->cfg.waves_per_group < 1u || b->cfg.waves_per_group > 4u ||
+      b->cfg.groups_per_cu < 1u) {
+    bbs_batch_destroy(b);
+    return bfail(nullptr, BBS_ERR_ARG, "invalid batch configuration");
+  }
+  b->jobs.resize(n_jobs);
+  uint64_t cells_total = 0, chains_total = 0, rows_total = 0;
+  std::vector<uint64_t> cells;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    std::string why;
+    int rc = fill_job(jobs[j], b->jobs[j], why);
+    if (rc) {
+      bbs_batch_destroy(b);
+      return bfail(nullptr, rc, "job %u: %s", j, why.c_str());
+    }
+    bbs::BatchJob &J = b->jobs[j];
+    J.cell_offset = (uint32_t)cells_total;
+    J.chain_offset = (uint32_t)chains_total;
+    J.row_offset = rows_total;
+    const uint32_t G = jobs[j].n_genes;
+    uint64_t gmask[2] = {G >= 64u ? ~0ull : ((1ull << G) - 1ull),
+                         G <= 64u ? 0ull : ((1ull << (G - 64u)) - 1ull)};
+    for (uint32_t c = 0; c < J.n_cells; c++) {
+      uint64_t q[4];
+      for (int wd = 0; wd < 2; wd++) {
+        uint64_t K = jobs[j].is_known[2u * c + wd] & gmask[wd];
+        uint64_t M = jobs[j].is_mutated[2u * c + wd];
+        q[wd] = K & M;
+        q[2 + wd] = K & ~M;
+      }
+      cells.insert(cells.end(), q, q + 4);
+    }
+    cells_total += J.n_cells;
+    chains_total += J.n_chains;
+    rows_total += (uint64_t)J.n_chains * J.n_nodes;
+    b->total_steps += (uint64_t)J.n_chains * (J.step_end - J.step_begin);
+  }
+  order_items(b);
+  hipDeviceProp_t prop;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    bbs_batch_destroy(b);
+    return bfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
+  }
+  b->n_cus = (uint32_t)prop.multiProcessorCount;
+  const size_t rows_b = rows_total * 16u, sc_b = chains_total * 4u;
+  int rc;
+  if ((rc = balloc(b, b->d_jobs, b->jobs.size() * sizeof(bbs::BatchJob))) ||
+      (rc = balloc(b, b->d_items, b->items.size() * 4u)) ||
+      (rc = balloc(b, b->d_counter, 4u)) ||
+      (rc = balloc(b, b->cells, cells.size() * 8u)) || (rc = balloc(b, b->am, rows_b)) ||
+      (rc = balloc(b, b->dm, rows_b)) || (rc = balloc(b, b->best_am, rows_b)) ||
+      (rc = balloc(b, b->best_dm, rows_b)) || (rc = balloc(b, b->beta, sc_b)) ||
+      (rc = balloc(b, b->score, sc_b)) || (rc = balloc(b, b->best_beta, sc_b)) ||
+      (rc = balloc(b, b->best_score, sc_b)) || (rc = balloc(b, b->best_step, sc_b))) {
+    std::string m = b->err;
+    bbs_batch_destroy(b);
+    return bfail(nullptr, rc, "%s", m.c_str());
+  }
+  e = hipMemcpy(b->d_jobs.p, b->jobs.data(), b->jobs.size() * sizeof(bbs::BatchJob),
+                hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_items.p, b->items.data(), b->items.size() * 4u,
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->cells.p, cells.data(), cells.size() * 8u, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    bbs_batch_destroy(b);
+    return bfail(nullptr, BBS_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return BBS_OK;
+}
+
+int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
+  if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
+  BHIP_TRY(b, hipSetDevice(b->device));
+  if (!b->exec) {
+    /* capture: initial states + counter reset, then the persistent kernel */
+    const bbs::BatchArgs a = batch_args(b);
+    const uint32_t groups = b->n_cus * b->cfg.groups_per_cu;
+    BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = bbs::launch_batch_init(a, b->stream);
+    hipError_t e2 = e1 == hipSuccess
+                        ? bbs::launch_batch(a, groups, b->cfg.waves_per_group, b->stream)
+                        : e1;
+    hipGraph_t g = nullptr;
+    hipError_t e3 = hipStreamEndCapture(b->stream, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return bfail(b, BBS_ERR_HIP, "graph capture failed: %s",
+                   hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+    }
+    b->graph = g;
+    BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  BHIP_TRY(b, hipEventRecord(b->ev0, b->stream));
+  BHIP_TRY(b, hipGraphLaunch(b->exec, b->stream));
+  BHIP_TRY(b, hipEventRecord(b->ev1, b->stream));
+  BHIP_TRY(b, hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (makespan_ms) *makespan_ms = ms;
+  b->ran = true;
+  return BBS_OK;
+}
+
+uint64_t bbs_batch_total_steps(const bbs_batch *b) { return b ? b->total_steps : 0u; }
+
+int bbs_batch_get_chain_results(bbs_batch *b, uint32_t job, float *best_score,
+                                uint32_t *best_step, float *final_score,
+                                float *final_beta) {
+  if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
+  if (job >= b->jobs.size()) return bfail(b, BBS_ERR_ARG, "job %u out of range", job);
+  if (!b->ran) return bfail(b, BBS_ERR_STATE, "batch has not run");
+  BHIP_TRY(b, hipSetDevice(b->device));
+  const bbs::BatchJob &J = b->jobs[job];
+  const size_t off = (size_t)J.chain_offset * 4u, n = (size_t)J.n_chains * 4u;
+  if (best_score)
+    BHIP_TRY(b, hipMemcpy(best_score, (char *)b->best_score.p + off, n, hipMemcpyDeviceToHost));
+  if (best_step)
+    BHIP_TRY(b, hipMemcpy(best_step, (char *)b->best_step.p + off, n, hipMemcpyDeviceToHost));
+  if (final_score)
+    BHIP_TRY(b, hipMemcpy(final_score, (char *)b->score.p + off, n, hipMemcpyDeviceToHost));
+  if (final_beta)
+    BHIP_TRY(b, hipMemcpy(final_beta, (char *)b->beta.p + off, n, hipMemcpyDeviceToHost));
+  return BBS_OK;
+}
+
+int bbs_batch_get_best(bbs_batch *b, uint32_t job, uint64_t *am, uint64_t *dm,
+                       float *beta, float *score) {
+  if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
+  if (job >= b->jobs.size()) return bfail(b, BBS_ERR_ARG, "job %u out of range", job);
+  const bbs::BatchJob &J = b->jobs[job];
+  if (J.step_end == 0u) return bfail(b, BBS_ERR_STATE, "job has no steps");
+  std::vector<float> s(J.n_chains), bb(J.n_chains);
+  std::vector<uint32_t> st(J.n_chains);
+  int rc = bbs_batch_get_chain_results(b, job, s.data(), st.data(), nullptr, nullptr);
+  if (rc) return rc;
+  BHIP_TRY(b, hipMemcpy(bb.data(), (char *)b->best_beta.p + (size_t)J.chain_offset * 4u,
+                        (size_t)J.n_chains * 4u, hipMemcpyDeviceToHost));
+  /* chain-major order as in bbs_get_best (src/Application.hpp:456) */
+  uint32_t best = 0u;
+  for (uint32_t c = 1; c < J.n_chains; c++)
+    if (st[c] != 0xffffffffu && s[c] > s[best]) best = c;
+  const size_t row_bytes = (size_t)J.n_nodes * 16u;
+  const size_t off = (size_t)(J.row_offset + (uint64_t)best * J.n_nodes) * 16u;
+  if (am)
+    BHIP_TRY(b, hipMemcpy(am, (char *)b->best_am.p + off, row_bytes, hipMemcpyDeviceToHost));
+  if (dm)
+    BHIP_TRY(b, hipMemcpy(dm, (char *)b->best_dm.p + off, row_bytes, hipMemcpyDeviceToHost));
+  if (beta) *beta = bb[best];
+  if (score) *score = s[best];
+  return BBS_OK;
+}
+
+int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
+                       bbs_batch_config *best, float *best_ms) {
+  if (!jobs || !best) return bfail(nullptr, BBS_ERR_ARG, "NULL argument");
+  std::vector<bbs_job> probe(jobs, jobs + n_jobs);
+  for (auto &j : probe)
+    if (j.params.n_steps > probe_steps) j.params.n_steps = probe_steps;
+  const uint32_t wpgs[3] = {1u, 2u, 4u};
+  const uint32_t waves_per_cu[3] = {4u, 8u, 16u};
+  float best_t = 1e30f;
+  bool found = false;
+  for (uint32_t order = 0; order < 2; order++) {
+    for (uint32_t wpg : wpgs) {
+      for (uint32_t wpc : waves_per_cu) {
+        bbs_batch_config c{wpg, wpc / wpg, order};
+        if (c.groups_per_cu == 0u) continue;
+        bbs_batch *b = nullptr;
+        int rc = bbs_batch_create(probe.data(), n_jobs, &c, &b);
+        if (rc) return rc;
+        float ms = 0.f, t = 1e30f;
+        for (int r = 0; r < 2 && rc == BBS_OK; r++) {
+          rc = bbs_batch_run(b, &ms);
+          if (r > 0 && ms < t) t = ms; /* first replay warms up */
+        }
+        bbs_batch_destroy(b);
+        if (rc) return rc;
+        if (t < best_t) {
+          best_t = t;
+          *best = c;
+          found = true;
+        }
+      }
+    }
+  }
+  if (!found) return bfail(nullptr, BBS_ERR_STATE, "no configuration ran");
+  if (best_ms) *best_ms = best_t;
   return BBS_OK;
 }
 

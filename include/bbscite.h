@@ -176,11 +176,64 @@ int bbs_bench_score_random(bbs_ctx *ctx, uint64_t seed, uint32_t n_trees,
 int bbs_random_trees_device(bbs_ctx *ctx, uint64_t seed, uint32_t purpose,
                             uint32_t n_trees, uint64_t *am, uint64_t *dm);
 
+/* Calibration of the int32 VALU roofline on this device: a kernel of
+ * independent v_and_b32 / v_bcnt_u32_b32 chains on every lane of every CU;
+ * returns the sustained int32 lane operations per second and wave64 VALU
+ * instructions per second. No reference counterpart. */
+int bbs_measure_valu_peak(bbs_ctx *ctx, double *int32_ops_per_s,
+                          double *valu_insts_per_s);
+
 /* Device values of expf(x), logf(x) and (float)log(1.0 - x) as the chain
  * kernel computes them (the libm calls of src/Application.hpp:454 and
  * src/TreeScorer.hpp:124,136-137), for comparison with the host libm. */
 int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
                   float *log_out, float *log1m_out);
+
+/* ---- batched persistent form --------------------------------------------
+ * Many independent datasets ("jobs") run in one graph-captured launch: a
+ * persistent kernel whose wavefronts pull (job, chain) items from a work
+ * counter, largest work first. Job j draws its random stream from
+ * (params.seed, job_tag, chain, step), so a job's results are identical to
+ * a plain bbs_run of the same data and parameters when job_tag is 0,
+ * whatever else is in the batch. No reference counterpart: the reference
+ * runs one dataset per FPGA invocation (src/main.cpp:159-161). */
+typedef struct bbs_job {
+  bbs_params params;             /* n_chains <= 65535                    */
+  uint32_t n_cells, n_genes;
+  const uint64_t *is_mutated;    /* n_cells rows, read at create time    */
+  const uint64_t *is_known;
+  uint32_t job_tag;              /* Philox job word; 0 = as bbs_run      */
+} bbs_job;
+
+/* Launch shape of the persistent kernel (chosen by bbs_batch_autotune). */
+typedef struct bbs_batch_config {
+  uint32_t waves_per_group; /* chains per workgroup: 1, 2 or 4          */
+  uint32_t groups_per_cu;   /* persistent workgroups per compute unit   */
+  uint32_t order;           /* 0: largest work first; 1: job order      */
+} bbs_batch_config;
+
+typedef struct bbs_batch bbs_batch;
+
+void bbs_batch_config_default(bbs_batch_config *cfg);
+/* cfg may be NULL (defaults). */
+int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs,
+                     const bbs_batch_config *cfg, bbs_batch **out);
+/* Runs every job from its initial states to completion (one hipGraph
+ * launch: initial trees, then the persistent kernel); rerunning replays the
+ * graph and reproduces the same results. */
+int bbs_batch_run(bbs_batch *b, float *makespan_ms);
+uint64_t bbs_batch_total_steps(const bbs_batch *b);
+int bbs_batch_get_best(bbs_batch *b, uint32_t job, uint64_t *am, uint64_t *dm,
+                       float *beta, float *score);
+int bbs_batch_get_chain_results(bbs_batch *b, uint32_t job, float *best_score,
+                                uint32_t *best_step, float *final_score,
+                                float *final_beta);
+const char *bbs_batch_last_error(const bbs_batch *b);
+void bbs_batch_destroy(bbs_batch *b);
+/* Times every candidate launch shape on the jobs with n_steps capped at
+ * probe_steps and returns the fastest. */
+int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
+                       bbs_batch_config *best, float *best_ms);
 
 /* Message of the last failure on ctx, or of the last failed bbs_create on
  * this thread when ctx is NULL. */

@@ -294,3 +294,57 @@ def test_errors_are_reported():
         ctx.begin()
         with pytest.raises(B.BbsError):
             ctx.advance(11)
+
+
+def _mix(n_jobs, seed0, steps, chains_range=(2, 6)):
+    rng = np.random.default_rng(seed0)
+    jobs = []
+    for i in range(n_jobs):
+        G = int(rng.integers(2, 128))
+        C = int(rng.integers(1, 129))
+        R = int(rng.integers(*chains_range))
+        m, _, _ = B.generate_dataset(seed0 + i, G, C, 1e-3, 0.2, 0.1)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=R, n_steps=steps, seed=seed0 + 100 + i,
+                             alpha_mean=1e-3, beta_mean=0.2 + 0.01 * (i % 5),
+                             prob_beta_change=0.1 * (i % 2))
+        jobs.append((p, mut, kno, G))
+    return jobs
+
+
+def test_batch_equals_plain_runs():
+    """Every job of a mixed batch gives exactly the plain bbs_run result."""
+    say("[batch]")
+    jobs = _mix(12, 40, 150)
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, groups_per_cu=2)) as b:
+        b.run()
+        batched = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
+        b.run()  # graph replay reproduces the run
+        again = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
+    for j, (p, mut, kno, G) in enumerate(jobs):
+        with B.Context(p, mut, kno, G) as ctx:
+            ctx.run()
+            plain = ctx.chain_results(), ctx.best()
+        for got in (batched[j], again[j]):
+            for x, y in zip(got[0], plain[0]):
+                assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                                      _bits(y) if y.dtype == np.float32 else y), j
+            assert np.array_equal(got[1][0], plain[1][0]) and np.array_equal(got[1][1], plain[1][1])
+            assert _bits(got[1][3]) == _bits(plain[1][3])
+
+
+def test_batch_configs_agree_and_autotune():
+    say("[batch configs]")
+    jobs = _mix(6, 77, 60)
+    ref = None
+    for wpg, gpc, order in ((1, 1, 0), (4, 1, 1), (2, 8, 0)):
+        with B.Batch(jobs, B.batch_config(wpg, gpc, order)) as b:
+            b.run()
+            res = [b.chain_results(j)[2] for j in range(len(jobs))]
+        if ref is None:
+            ref = res
+        else:
+            for x, y in zip(ref, res):
+                assert np.array_equal(_bits(x), _bits(y))
+    cfg, ms = B.batch_autotune(jobs, probe_steps=20)
+    assert cfg.waves_per_group in (1, 2, 4) and ms > 0
