@@ -103,8 +103,9 @@ class Job(C.Structure):
 class BatchConfig(C.Structure):
     _fields_ = [
         ("waves_per_group", C.c_uint32),
-        ("groups_per_cu", C.c_uint32),
+        ("n_streams", C.c_uint32),
         ("order", C.c_uint32),
+        ("persistent", C.c_uint32),
     ]
 
 
@@ -457,13 +458,13 @@ def _bcheck(rc, b=None):
         raise BbsError(rc, msg.decode() if msg else "")
 
 
-def batch_config(waves_per_group=None, groups_per_cu=None, order=None):
+def batch_config(waves_per_group=None, n_streams=None, order=None):
     c = BatchConfig()
     lib().bbs_batch_config_default(C.byref(c))
     if waves_per_group is not None:
         c.waves_per_group = waves_per_group
-    if groups_per_cu is not None:
-        c.groups_per_cu = groups_per_cu
+    if n_streams is not None:
+        c.n_streams = n_streams
     if order is not None:
         c.order = order
     return c

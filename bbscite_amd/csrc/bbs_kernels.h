@@ -94,6 +94,10 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
 hipError_t launch_batch_init(const BatchArgs &a, hipStream_t stream);
+/* items [first, first + count) all have the given shape code */
+hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
+                              uint32_t count, uint32_t waves_per_group,
+                              hipStream_t stream);
 hipError_t launch_batch(const BatchArgs &a, uint32_t groups, uint32_t waves_per_group,
                         hipStream_t stream);
 hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,

@@ -455,6 +455,54 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
   }
 }
 
+/*
+ * Batched form with one wavefront per (job, chain) item: the items of one
+ * register layout (shape) are launched together, largest work first, so
+ * the hardware dispatcher hands the next item to whichever SIMD frees up
+ * first. waves_per_group wavefronts share a workgroup and run independent
+ * items (run_chain has no workgroup barrier).
+ */
+template <int NS, int CS, int W>
+__global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t first,
+                                                          uint32_t count) {
+  __shared__ __align__(16) float lds_all[4][64 * CS];
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t k = blockIdx.x * (blockDim.x >> 6) + wave;
+  if (k >= count) return;
+  const uint32_t packed = a.items[first + k];
+  const uint32_t j = packed >> 16, c = packed & 0xffffu;
+  const BatchJob &J = a.jobs[j];
+  const uint64_t rows = 2ull * ((uint64_t)J.row_offset + (uint64_t)c * J.n_nodes);
+  const uint32_t gc = J.chain_offset + c;
+  ChainView v;
+  v.cells = a.cells + 4ull * J.cell_offset;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + gc;
+  v.score = a.score + gc;
+  v.best_beta = a.best_beta + gc;
+  v.best_score = a.best_score + gc;
+  v.best_step = a.best_step + gc;
+  v.trace = nullptr;
+  v.seed = J.seed;
+  v.job = J.job_tag;
+  v.chain = c;
+  v.step_begin = J.step_begin;
+  v.step_end = J.step_end;
+  v.n_cells = J.n_cells;
+  v.n_nodes = J.n_nodes;
+  v.is_first_chain = c == 0u;
+  v.score_init = J.score_init != 0u;
+  v.L00 = J.L00;
+  v.L10 = J.L10;
+  v.gamma = J.gamma;
+  v.prior = J.prior;
+  v.probs = J.probs;
+  run_chain<NS, CS, W, false>(v, lds_all[wave]);
+}
+
 /* Initial states of every (job, chain) of a batch: uniform random trees
  * from each job's own seed, as the plain form draws them. */
 __global__ __launch_bounds__(64) void batch_init_kernel(BatchArgs a) {
@@ -878,6 +926,23 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
   }
   BBS_SHAPES(X)
 #undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
+                              uint32_t count, uint32_t waves_per_group,
+                              hipStream_t stream) {
+  if (count == 0u) return hipSuccess;
+  if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
+  const uint32_t groups = (count + waves_per_group - 1u) / waves_per_group;
+#define XB(NS_, CS_, W_)                                                       \
+  if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_)) {                      \
+    hipLaunchKernelGGL((batch_items_kernel<NS_, CS_, W_>), dim3(groups),       \
+                       dim3(64u * waves_per_group), 0, stream, a, first, count); \
+    return hipGetLastError();                                                  \
+  }
+  BBS_SHAPES(XB)
+#undef XB
   return hipErrorInvalidValue;
 }
 

@@ -837,10 +837,17 @@ int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
 
 struct bbs_batch {
   int device = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;           /* stream 0 */
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t side_done[3] = {nullptr, nullptr, nullptr};
   std::vector<bbs::BatchJob> jobs;
   std::vector<uint32_t> items;
+  struct Range {
+    uint32_t shape, first, count;
+    double work;
+  };
+  std::vector<Range> ranges; /* items of one shape, heaviest range first */
   bbs_batch_config cfg{};
   uint32_t n_cus = 0;
   uint64_t total_steps = 0;
@@ -953,18 +960,35 @@ double step_cost(const bbs::BatchJob &J) {
 
 void order_items(bbs_batch *b) {
   b->items.clear();
+  b->ranges.clear();
   std::vector<uint32_t> order(b->jobs.size());
   for (uint32_t j = 0; j < order.size(); j++) order[j] = j;
-  if (b->cfg.order == 0u) {
-    /* largest chain first: a greedy longest-processing-time schedule */
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-      const auto &A = b->jobs[x], &B = b->jobs[y];
-      return step_cost(A) * (A.step_end - A.step_begin) >
-             step_cost(B) * (B.step_end - B.step_begin);
-    });
+  auto work = [&](uint32_t x) {
+    const auto &A = b->jobs[x];
+    return step_cost(A) * (A.step_end - A.step_begin);
+  };
+  /* group by register layout; inside a group largest chain first (a greedy
+   * longest-processing-time schedule for the hardware dispatcher) unless
+   * order == 1 (job order) */
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    if (b->jobs[x].shape != b->jobs[y].shape) return b->jobs[x].shape < b->jobs[y].shape;
+    return b->cfg.order == 0u ? work(x) > work(y) : false;
+  });
+  for (uint32_t idx = 0; idx < order.size();) {
+    const uint32_t shape = b->jobs[order[idx]].shape;
+    bbs_batch::Range r{shape, (uint32_t)b->items.size(), 0u, 0.0};
+    for (; idx < order.size() && b->jobs[order[idx]].shape == shape; idx++) {
+      const uint32_t j = order[idx];
+      for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
+      r.work += work(j) * b->jobs[j].n_chains;
+    }
+    r.count = (uint32_t)b->items.size() - r.first;
+    if (r.count) b->ranges.push_back(r);
   }
-  for (uint32_t j : order)
-    for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
+  std::stable_sort(b->ranges.begin(), b->ranges.end(),
+                   [](const bbs_batch::Range &x, const bbs_batch::Range &y) {
+                     return x.work > y.work;
+                   });
 }
 
 bbs::BatchArgs batch_args(bbs_batch *b) {
@@ -993,8 +1017,9 @@ extern "C" {
 void bbs_batch_config_default(bbs_batch_config *cfg) {
   if (!cfg) return;
   cfg->waves_per_group = 1u;
-  cfg->groups_per_cu = 8u;
+  cfg->n_streams = 4u;
   cfg->order = 0u;
+  cfg->persistent = 0u;
 }
 
 const char *bbs_batch_last_error(const bbs_batch *b) {
@@ -1014,6 +1039,11 @@ void bbs_batch_destroy(bbs_batch *b) {
   for (DevBuf *d : bufs) d->release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (int i = 0; i < 3; i++) {
+    if (b->side[i]) (void)hipStreamSynchronize(b->side[i]);
+    if (b->side_done[i]) (void)hipEventDestroy(b->side_done[i]);
+    if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+  }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -1039,7 +1069,7 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   else
     bbs_batch_config_default(&b->cfg);
   if (b->cfg.waves_per_group < 1u || b->cfg.waves_per_group > 4u ||
-      b->cfg.groups_per_cu < 1u) {
+      b->cfg.n_streams < 1u || b->cfg.n_streams > 4u) {
     bbs_batch_destroy(b);
     return bfail(nullptr, BBS_ERR_ARG, "invalid batch configuration");
   }
@@ -1082,6 +1112,10 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  for (uint32_t i = 0; i + 1u < b->cfg.n_streams && e == hipSuccess; i++) {
+    e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->side_done[i], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     bbs_batch_destroy(b);
     return bfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
@@ -1118,30 +1152,58 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
 
 int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
   if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
+  if (b->cfg.persistent)
+    return bfail(b, BBS_ERR_STATE,
+                 "the persistent graph-captured form is switched off: its first GPU "
+                 "test hung and the cause has not been found");
   BHIP_TRY(b, hipSetDevice(b->device));
-  if (!b->exec) {
-    /* capture: initial states + counter reset, then the persistent kernel */
-    const bbs::BatchArgs a = batch_args(b);
-    const uint32_t groups = b->n_cus * b->cfg.groups_per_cu;
-    BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = bbs::launch_batch_init(a, b->stream);
-    hipError_t e2 = e1 == hipSuccess
-                        ? bbs::launch_batch(a, groups, b->cfg.waves_per_group, b->stream)
-                        : e1;
-    hipGraph_t g = nullptr;
-    hipError_t e3 = hipStreamEndCapture(b->stream, &g);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return bfail(b, BBS_ERR_HIP, "graph capture failed: %s",
-                   hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
-    }
-    b->graph = g;
-    BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  }
+  const bbs::BatchArgs a = batch_args(b);
+  /* initial states: uniform random trees from each job's seed, exactly as
+   * bbs_begin draws them; no maxima yet */
+  std::vector<uint32_t> steps(a.n_items ? b->jobs.back().chain_offset + b->jobs.back().n_chains : 0u,
+                              0xffffffffu);
+  std::vector<float> ninf(steps.size(), -INFINITY);
   BHIP_TRY(b, hipEventRecord(b->ev0, b->stream));
-  BHIP_TRY(b, hipGraphLaunch(b->exec, b->stream));
-  BHIP_TRY(b, hipEventRecord(b->ev1, b->stream));
-  BHIP_TRY(b, hipEventSynchronize(b->ev1));
+  for (const bbs::BatchJob &J : b->jobs) {
+    bbs::TreeGenArgs g{};
+    g.am = a.am + 2ull * J.row_offset;
+    g.dm = a.dm + 2ull * J.row_offset;
+    g.beta = a.beta + J.chain_offset;
+    g.beta_value = J.beta_mean;
+    g.seed = J.seed;
+    g.job = J.job_tag;
+    g.chain_offset = 0u;
+    g.purpose = BBS_PURPOSE_INIT_TREE;
+    g.n_trees = J.n_chains;
+    g.n_nodes = J.n_nodes;
+    BHIP_TRY(b, bbs::launch_random_trees(g, b->stream));
+  }
+  BHIP_TRY(b, hipMemcpyAsync(b->best_step.p, steps.data(), steps.size() * 4u,
+                             hipMemcpyHostToDevice, b->stream));
+  BHIP_TRY(b, hipMemcpyAsync(b->best_score.p, ninf.data(), ninf.size() * 4u,
+                             hipMemcpyHostToDevice, b->stream));
+  BHIP_TRY(b, hipMemcpyAsync(b->best_beta.p, ninf.data(), ninf.size() * 4u,
+                             hipMemcpyHostToDevice, b->stream));
+  /* the shape ranges, heaviest first, spread over the streams */
+  hipEvent_t ready = nullptr;
+  BHIP_TRY(b, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ready, b->stream);
+  const uint32_t ns = b->cfg.n_streams;
+  for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++)
+    e = hipStreamWaitEvent(b->side[i], ready, 0);
+  for (size_t r = 0; r < b->ranges.size() && e == hipSuccess; r++) {
+    const auto &R = b->ranges[r];
+    hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
+    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, b->cfg.waves_per_group, st);
+  }
+  for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++) {
+    e = hipEventRecord(b->side_done[i], b->side[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->side_done[i], 0);
+  }
+  if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+  (void)hipEventDestroy(ready);
+  if (e != hipSuccess) return bfail(b, BBS_ERR_HIP, "batch run failed: %s", hipGetErrorString(e));
   float ms = 0.f;
   BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
   if (makespan_ms) *makespan_ms = ms;
@@ -1205,21 +1267,20 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
   for (auto &j : probe)
     if (j.params.n_steps > probe_steps) j.params.n_steps = probe_steps;
   const uint32_t wpgs[3] = {1u, 2u, 4u};
-  const uint32_t waves_per_cu[3] = {4u, 8u, 16u};
+  const uint32_t streams[3] = {1u, 2u, 4u};
   float best_t = 1e30f;
   bool found = false;
   for (uint32_t order = 0; order < 2; order++) {
     for (uint32_t wpg : wpgs) {
-      for (uint32_t wpc : waves_per_cu) {
-        bbs_batch_config c{wpg, wpc / wpg, order};
-        if (c.groups_per_cu == 0u) continue;
+      for (uint32_t nst : streams) {
+        bbs_batch_config c{wpg, nst, order, 0u};
         bbs_batch *b = nullptr;
         int rc = bbs_batch_create(probe.data(), n_jobs, &c, &b);
         if (rc) return rc;
         float ms = 0.f, t = 1e30f;
         for (int r = 0; r < 2 && rc == BBS_OK; r++) {
           rc = bbs_batch_run(b, &ms);
-          if (r > 0 && ms < t) t = ms; /* first replay warms up */
+          if (r > 0 && ms < t) t = ms; /* the first run warms up */
         }
         bbs_batch_destroy(b);
         if (rc) return rc;

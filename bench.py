@@ -133,7 +133,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
             bests_plain.append(ctx.best()[3])
     plain_steps = sum(p.n_chains * p.n_steps for p, *_ in jobs_plain)
     cfg, probe_ms = B.batch_autotune(jobs_plain, probe_steps)
-    log(f"[bench] batch autotune: wpg={cfg.waves_per_group} gpc={cfg.groups_per_cu} "
+    log(f"[bench] batch autotune: wpg={cfg.waves_per_group} streams={cfg.n_streams} "
         f"order={cfg.order} ({probe_ms:.1f} ms at {probe_steps} steps)")
     with B.Batch(jobs_plain, cfg) as b:
         b.run()
@@ -154,7 +154,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
         "batched_ms": ms,
         "best_ll_identical_to_plain": same,
         "autotuned": {"waves_per_group": cfg.waves_per_group,
-                      "groups_per_cu": cfg.groups_per_cu, "order": cfg.order},
+                      "n_streams": cfg.n_streams, "order": cfg.order},
     }
 
 

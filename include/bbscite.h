@@ -189,10 +189,11 @@ int bbs_measure_valu_peak(bbs_ctx *ctx, double *int32_ops_per_s,
 int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
                   float *log_out, float *log1m_out);
 
-/* ---- batched persistent form --------------------------------------------
- * Many independent datasets ("jobs") run in one graph-captured launch: a
- * persistent kernel whose wavefronts pull (job, chain) items from a work
- * counter, largest work first. Job j draws its random stream from
+/* ---- batched form ---------------------------------------------------------
+ * Many independent datasets ("jobs") run together: the (job, chain) items
+ * of each register layout go out as one launch, largest work first, and the
+ * layouts run side by side on up to four streams. Job j draws its random
+ * stream from
  * (params.seed, job_tag, chain, step), so a job's results are identical to
  * a plain bbs_run of the same data and parameters when job_tag is 0,
  * whatever else is in the batch. No reference counterpart: the reference
@@ -205,11 +206,13 @@ typedef struct bbs_job {
   uint32_t job_tag;              /* Philox job word; 0 = as bbs_run      */
 } bbs_job;
 
-/* Launch shape of the persistent kernel (chosen by bbs_batch_autotune). */
+/* Launch shape (chosen by bbs_batch_autotune). */
 typedef struct bbs_batch_config {
   uint32_t waves_per_group; /* chains per workgroup: 1, 2 or 4          */
-  uint32_t groups_per_cu;   /* persistent workgroups per compute unit   */
+  uint32_t n_streams;       /* streams the layouts are spread over, 1-4 */
   uint32_t order;           /* 0: largest work first; 1: job order      */
+  uint32_t persistent;      /* 1: persistent graph-captured kernel;
+                               switched off, bbs_batch_run rejects it   */
 } bbs_batch_config;
 
 typedef struct bbs_batch bbs_batch;
@@ -218,9 +221,8 @@ void bbs_batch_config_default(bbs_batch_config *cfg);
 /* cfg may be NULL (defaults). */
 int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs,
                      const bbs_batch_config *cfg, bbs_batch **out);
-/* Runs every job from its initial states to completion (one hipGraph
- * launch: initial trees, then the persistent kernel); rerunning replays the
- * graph and reproduces the same results. */
+/* Runs every job from its initial states to completion; rerunning
+ * reproduces the same results. */
 int bbs_batch_run(bbs_batch *b, float *makespan_ms);
 uint64_t bbs_batch_total_steps(const bbs_batch *b);
 int bbs_batch_get_best(bbs_batch *b, uint32_t job, uint64_t *am, uint64_t *dm,

@@ -12,6 +12,11 @@ if TESTS not in sys.path:
 
 
 def pytest_configure(config):
+    # A GPU call that never returns should name itself: dump every thread's
+    # stack well before an outer watchdog would kill a silent run.
+    import faulthandler
+
+    faulthandler.dump_traceback_later(150, repeat=True)
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     # Build only what is missing: the oracle (gcc) and the HIP library (hipcc).
     from tests import _oracle  # noqa: F401

@@ -312,15 +312,46 @@ def _mix(n_jobs, seed0, steps, chains_range=(2, 6)):
     return jobs
 
 
+PERSISTENT_HANG = (
+    "the persistent graph-captured batched kernel hung on its first GPU run "
+    "(no output for 180 s); its cause has not been found, so that path is "
+    "switched off (bbs_batch_run rejects persistent = 1) and stays off the GPU"
+)
+
+
+@pytest.mark.skip(reason=PERSISTENT_HANG)
 def test_batch_equals_plain_runs():
-    """Every job of a mixed batch gives exactly the plain bbs_run result."""
-    say("[batch]")
+    """(persistent form) every job of a mixed batch gives the plain result."""
+
+
+@pytest.mark.skip(reason=PERSISTENT_HANG)
+def test_batch_configs_agree_and_autotune():
+    """(persistent form) launch shapes agree; autotuner picks one."""
+
+
+def test_batch_persistent_mode_is_rejected():
+    jobs = _mix(1, 5, 10)
+    with B.Batch(jobs, B.batch_config()) as b:
+        pass
+    cfg = B.batch_config()
+    cfg.persistent = 1
+    with B.Batch(jobs, cfg) as b:
+        with pytest.raises(B.BbsError) as e:
+            b.run()
+        assert e.value.code == B.ERR_STATE
+
+
+def test_batched_items_equal_plain_runs():
+    """Every job of a mixed batch (one wave per item, per-layout launches)
+    gives exactly the plain bbs_run result, on every rerun."""
+    say("[batched items]")
     jobs = _mix(12, 40, 150)
-    with B.Batch(jobs, B.batch_config(waves_per_group=2, groups_per_cu=2)) as b:
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4)) as b:
         b.run()
         batched = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
-        b.run()  # graph replay reproduces the run
+        b.run()
         again = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
+    say("[batched items] plain runs")
     for j, (p, mut, kno, G) in enumerate(jobs):
         with B.Context(p, mut, kno, G) as ctx:
             ctx.run()
@@ -333,12 +364,12 @@ def test_batch_equals_plain_runs():
             assert _bits(got[1][3]) == _bits(plain[1][3])
 
 
-def test_batch_configs_agree_and_autotune():
-    say("[batch configs]")
+def test_batched_items_configs_agree_and_autotune():
+    say("[batched configs]")
     jobs = _mix(6, 77, 60)
     ref = None
-    for wpg, gpc, order in ((1, 1, 0), (4, 1, 1), (2, 8, 0)):
-        with B.Batch(jobs, B.batch_config(wpg, gpc, order)) as b:
+    for wpg, nst, order in ((1, 1, 0), (4, 2, 1), (2, 4, 0)):
+        with B.Batch(jobs, B.batch_config(wpg, nst, order)) as b:
             b.run()
             res = [b.chain_results(j)[2] for j in range(len(jobs))]
         if ref is None:
@@ -347,4 +378,4 @@ def test_batch_configs_agree_and_autotune():
             for x, y in zip(ref, res):
                 assert np.array_equal(_bits(x), _bits(y))
     cfg, ms = B.batch_autotune(jobs, probe_steps=20)
-    assert cfg.waves_per_group in (1, 2, 4) and ms > 0
+    assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 2, 4) and ms > 0
