@@ -159,22 +159,38 @@ __device__ __forceinline__ float vmax(float a, float b) {
   return r;
 }
 
+/* Cell slots per lane: CS = 1 holds cells 0..63; CS = 2 cells 0..127 in two
+ * slots; CS = 3 cells 0..95 with the second slot "paired": lanes l and
+ * l + 32 both hold cell 64 + l % 32 and score different nodes, so the 32
+ * cells above 64 cost half a slot. */
+template <int CS>
+struct Slots {
+  static constexpr int n = CS == 1 ? 1 : 2;
+};
+
 /* Mutation data of the cells this lane scores. */
 template <int CS, int W>
 struct Cells {
-  uint32_t km[CS][W];  /* K & M       */
-  uint32_t knm[CS][W]; /* K & ~M      */
-  float fb[CS];        /* |K & M|     */
-  float fa[CS];        /* |K & ~M|    */
+  uint32_t km[Slots<CS>::n][W];  /* K & M       */
+  uint32_t knm[Slots<CS>::n][W]; /* K & ~M      */
+  float fb[Slots<CS>::n];        /* |K & M|     */
+  float fa[Slots<CS>::n];        /* |K & ~M|    */
 };
+
+/* Cell held by this lane in slot j. */
+template <int CS>
+__device__ __forceinline__ uint32_t slot_cell(int j) {
+  if (j == 0) return lane_id();
+  return CS == 3 ? 64u + (lane_id() & 31u) : 64u + lane_id();
+}
 
 /* cells: [n_cells][4] words (K&M lo, K&M hi, K&~M lo, K&~M hi). */
 template <int CS, int W>
 __device__ __forceinline__ void load_cells(const uint64_t *cells,
                                            uint32_t n_cells, Cells<CS, W> &c) {
 #pragma unroll
-  for (int j = 0; j < CS; j++) {
-    uint32_t cell = lane_id() + 64u * (uint32_t)j;
+  for (int j = 0; j < Slots<CS>::n; j++) {
+    uint32_t cell = slot_cell<CS>(j);
     uint64_t q[4] = {0ull, 0ull, 0ull, 0ull};
     if (cell < n_cells) {
 #pragma unroll
@@ -197,6 +213,14 @@ __device__ __forceinline__ void load_cells(const uint64_t *cells,
   }
 }
 
+/* Per-wave LDS: a copy of the tree's ancestor rows (one uint4 per node,
+ * read as broadcasts by the scorer) and the per-cell maxima for the ordered
+ * sum. */
+struct WaveLds {
+  uint4 rows[128];
+  float sums[192];
+};
+
 /* The four log-probabilities of src/TreeScorer.hpp:99-106,136-137. */
 struct LogProbs {
   float L00, L01, L10, L11;
@@ -218,51 +242,67 @@ __device__ __forceinline__ float pair_score(uint32_t n01, uint32_t n11,
   return s;
 }
 
+/* The two popcounts of one (cell, node) pair: n11 = |K&M&A|,
+ * n01 = |K&~M&A|. */
+template <int W>
+__device__ __forceinline__ void pair_counts(const uint32_t (&km)[W],
+                                            const uint32_t (&knm)[W], const uint4 &A,
+                                            uint32_t &n01, uint32_t &n11) {
+  n11 = bcnt_acc(km[0] & A.x, 0u);
+  n01 = bcnt_acc(knm[0] & A.x, 0u);
+  if (W > 1) {
+    n11 = bcnt_acc(km[W > 1 ? 1 : 0] & A.y, n11);
+    n01 = bcnt_acc(knm[W > 1 ? 1 : 0] & A.y, n01);
+  }
+  if (W > 2) {
+    n11 = bcnt_acc(km[W > 2 ? 2 : 0] & A.z, n11);
+    n01 = bcnt_acc(knm[W > 2 ? 2 : 0] & A.z, n01);
+  }
+  if (W > 3) {
+    n11 = bcnt_acc(km[W > 3 ? 3 : 0] & A.w, n11);
+    n01 = bcnt_acc(knm[W > 3 ? 3 : 0] & A.w, n01);
+  }
+}
+
 /*
  * TreeScorer::logscore_tree without the beta prior (src/TreeScorer.hpp:
  * 131-205): per cell the maximum over nodes (a NaN at node 0 sticks, later
  * NaNs are ignored, as std::max(old, new) does), then the sum over cells in
- * order. Returns the same value in every lane. lds: 64*CS floats of this
- * wave. With COUNTS, n01/n11 of the first maximal node are returned per slot.
+ * order. Lanes hold cells; the tree's rows are copied to S.rows and read as
+ * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
+ * or 2 only), n01/n11 of the first maximal node are returned per slot.
  */
 template <int NS, int CS, int W, bool COUNTS>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
-           uint32_t n_cells, const LogProbs &L, float *lds,
-           uint32_t (&arg01)[CS], uint32_t (&arg11)[CS]) {
-  float best[CS];
-  bool nan0[CS];
-  /* node 0 seeds the maximum (src/TreeScorer.hpp:187-188) */
-  auto first = [&](const uint32_t (&A)[4]) {
+           uint32_t n_cells, const LogProbs &L, WaveLds &S,
+           uint32_t (&arg01)[2], uint32_t (&arg11)[2]) {
+  static_assert(!COUNTS || CS != 3, "counts need full cell slots");
+  constexpr int SL = Slots<CS>::n;
 #pragma unroll
-    for (int j = 0; j < CS; j++) {
-      uint32_t n11 = 0, n01 = 0;
+  for (int k = 0; k < NS; k++) {
+    const uint64_t lo = dm[k][0], hi = NS > 1 ? dm[k][NS - 1] : 0ull;
+    S.rows[lane_id() + 64u * (uint32_t)k] =
+        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+  }
+  wave_lds_sync();
+  float best[2];
+  bool nan0[2] = {false, false};
+  /* one node for the full slots (node 0 seeds the maximum) */
+  auto full = [&](const uint4 &A, bool first) {
 #pragma unroll
-      for (int w = 0; w < W; w++) {
-        n11 = bcnt_acc(c.km[j][w] & A[w], n11);
-        n01 = bcnt_acc(c.knm[j][w] & A[w], n01);
-      }
-      float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      best[j] = s;
-      nan0[j] = s != s;
-      if (COUNTS) {
-        arg01[j] = n01;
-        arg11[j] = n11;
-      }
-    }
-  };
-  /* later nodes replace it only when strictly greater (:189-190) */
-  auto rest = [&](const uint32_t (&A)[4]) {
-#pragma unroll
-    for (int j = 0; j < CS; j++) {
-      uint32_t n11 = 0, n01 = 0;
-#pragma unroll
-      for (int w = 0; w < W; w++) {
-        n11 = bcnt_acc(c.km[j][w] & A[w], n11);
-        n01 = bcnt_acc(c.knm[j][w] & A[w], n01);
-      }
-      float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      if (COUNTS) {
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      uint32_t n01, n11;
+      pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
+      const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
+      if (first) {
+        best[j] = s;
+        nan0[j] = s != s;
+        if (COUNTS) {
+          arg01[j] = n01;
+          arg11[j] = n11;
+        }
+      } else if (COUNTS) {
         if (s > best[j]) {
           best[j] = s;
           arg01[j] = n01;
@@ -273,44 +313,60 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
       }
     }
   };
-  auto words = [&](uint32_t k, uint32_t lane, uint32_t (&A)[4]) {
-    A[0] = readlane32((uint32_t)dm[k][0], lane);
-    A[1] = W > 1 ? readlane32((uint32_t)(dm[k][0] >> 32), lane) : 0u;
-    A[2] = W > 2 ? readlane32((uint32_t)dm[k][NS - 1], lane) : 0u;
-    A[3] = W > 3 ? readlane32((uint32_t)(dm[k][NS - 1] >> 32), lane) : 0u;
-  };
-  {
-    uint32_t A[4];
-    words(0u, 0u, A);
-    first(A);
-  }
-  const uint32_t n_lo = n_nodes < 64u ? n_nodes : 64u;
-  for (uint32_t v = 1; v < n_lo; v++) {
-    uint32_t A[4];
-    words(0u, v, A);
-    rest(A);
-  }
-  if (NS > 1) {
-    for (uint32_t v = 64u; v < n_nodes; v++) {
-      uint32_t A[4];
-      words((uint32_t)NS - 1u, v - 64u, A);
-      rest(A);
+  /* the paired slot: lanes < 32 score node v, lanes >= 32 node v + 1 */
+  auto half = [&](const uint4 &A, bool first) {
+    uint32_t n01, n11;
+    pair_counts<W>(c.km[1], c.knm[1], A, n01, n11);
+    const float s = pair_score(n01, n11, c.fa[1], c.fb[1], L);
+    if (first) {
+      best[1] = s;
+      nan0[1] = lane_id() < 32u && s != s;
+    } else {
+      best[1] = vmax(best[1], s);
     }
+  };
+  const uint32_t hi_half = lane_id() >> 5; /* 0 or 1 */
+  if (CS == 3) {
+    const uint32_t last = n_nodes - 1u;
+    full(S.rows[0], true);
+    if (n_nodes > 1u) full(S.rows[1], false);
+    half(S.rows[hi_half < last ? hi_half : last], true);
+    for (uint32_t v = 2; v < n_nodes; v += 2) {
+      const uint32_t vh = v + hi_half;
+      const uint4 Ah = S.rows[vh < last ? vh : last];
+      full(S.rows[v], false);
+      if (v + 1u < n_nodes) full(S.rows[v + 1u], false);
+      half(Ah, false);
+    }
+  } else {
+    full(S.rows[0], true);
+    for (uint32_t v = 1; v < n_nodes; v++) full(S.rows[v], false);
   }
-#pragma unroll
-  for (int j = 0; j < CS; j++)
-    lds[lane_id() + 64u * (uint32_t)j] = nan0[j] ? __int_as_float(0x7fc00000) : best[j];
+  const float qnan = __int_as_float(0x7fc00000);
+  S.sums[lane_id()] = nan0[0] ? qnan : best[0];
+  if (SL > 1) S.sums[64u + lane_id()] = nan0[1] ? qnan : best[1];
   wave_lds_sync();
   float T = 0.0f;
+  const uint32_t n_lo = n_cells < 64u ? n_cells : 64u;
   uint32_t cc = 0;
-  for (; cc + 4u <= n_cells; cc += 4u) {
-    float4 q = *reinterpret_cast<const float4 *>(lds + cc);
+  for (; cc + 4u <= n_lo; cc += 4u) {
+    const float4 q = *reinterpret_cast<const float4 *>(S.sums + cc);
     T = T + q.x;
     T = T + q.y;
     T = T + q.z;
     T = T + q.w;
   }
-  for (; cc < n_cells; cc++) T = T + lds[cc];
+  for (; cc < n_lo; cc++) T = T + S.sums[cc];
+  if (CS == 3) {
+    /* cell 64 + i: the maximum over even nodes (lane i) and odd nodes (lane
+     * 32 + i); a NaN from node 0 only comes from the even half */
+    for (; cc < n_cells; cc++) {
+      const float x = S.sums[cc], y = S.sums[cc + 32u];
+      T = T + (x != x ? x : vmax(x, y));
+    }
+  } else {
+    for (; cc < n_cells; cc++) T = T + S.sums[cc];
+  }
   wave_lds_sync();
   return rfl_f(T);
 }

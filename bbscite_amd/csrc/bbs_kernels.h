@@ -73,33 +73,32 @@ struct BatchJob {
   bbs_move_probs probs;
 };
 
-/* items: (job << 16 | chain) in scheduling order; counter: work counter
- * (reset to 0 by the init kernel). */
+/* items: (job << 16 | chain) in scheduling order. */
 struct BatchArgs {
   const BatchJob *jobs;
   const uint32_t *items;
   uint32_t n_items;
-  uint32_t *counter;
   const uint64_t *cells;
   uint64_t *am, *dm, *best_am, *best_dm;
   float *beta, *score, *best_beta, *best_score;
   uint32_t *best_step;
 };
 
-void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w);
+/* Register layout for a dataset: NS node slots, CS cell slots (3 = paired
+ * half slot for 65..96 cells, used when paired is true), W gene words. */
+void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
+              bool paired);
 hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream);
 /* cells32: [128][10] words for the lane-per-tree kernel (used when counts
  * are not requested), or null to use the lane-per-cell kernel. */
 hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
-hipError_t launch_batch_init(const BatchArgs &a, hipStream_t stream);
 /* items [first, first + count) all have the given shape code */
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
                               uint32_t count, uint32_t waves_per_group,
                               hipStream_t stream);
-hipError_t launch_batch(const BatchArgs &a, uint32_t groups, uint32_t waves_per_group,
-                        hipStream_t stream);
+
 hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
                             uint32_t iters, hipStream_t stream);
 hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,

@@ -23,9 +23,13 @@ namespace bbs {
 /* The register layouts instantiated: (NS node slots, CS cell slots, W gene
  * words) for every gene/cell count up to 127 x 128. */
 #define BBS_SHAPES(X)                                                          \
+  X(1, 1, 1) X(1, 1, 2) X(1, 2, 1) X(1, 2, 2) X(1, 3, 1) X(1, 3, 2) X(2, 1, 2) \
+      X(2, 1, 3) X(2, 1, 4) X(2, 2, 2) X(2, 2, 3) X(2, 2, 4) X(2, 3, 2)         \
+      X(2, 3, 3) X(2, 3, 4)
+/* the layouts with full cell slots (the counting scorer) */
+#define BBS_FULL_SHAPES(X)                                                     \
   X(1, 1, 1) X(1, 1, 2) X(1, 2, 1) X(1, 2, 2) X(2, 1, 2) X(2, 1, 3) X(2, 1, 4) \
       X(2, 2, 2) X(2, 2, 3) X(2, 2, 4)
-
 #define BBS_SHAPE_CODE(NS_, CS_, W_) ((NS_)*100 + (CS_)*10 + (W_))
 
 __device__ void random_tree_wave(const TreeGenArgs &a, uint32_t tree);
@@ -164,7 +168,7 @@ struct ChainView {
 /* Steps [step_begin, step_end) of one chain: the WorkKernel loop body of
  * src/Application.hpp:412-487 with the state resident in VGPRs. */
 template <int NS, int CS, int W, bool TRACE>
-__device__ __forceinline__ void run_chain(const ChainView &a, float *lds) {
+__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
 
@@ -189,7 +193,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, float *lds) {
   L.L01 = bbs_logf(cur_beta);
   L.L11 = bbs_logf_of_double(1.0 - (double)cur_beta);
   float cur_prior = logscore_beta(a.prior, cur_beta);
-  uint32_t unused01[CS], unused11[CS];
+  uint32_t unused01[2], unused11[2];
   float cur_score;
   if (a.score_init)
     cur_score = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds,
@@ -360,7 +364,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, float *lds) {
 /* One dataset per launch, one wavefront per chain (the drop-in form). */
 template <int NS, int CS, int W, bool TRACE>
 __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
-  __shared__ __align__(16) float lds[64 * CS];
+  __shared__ WaveLds lds;
   const uint32_t chain = blockIdx.x;
   if (chain >= a.n_chains) return;
   const uint64_t rows = 2ull * chain * a.n_nodes;
@@ -397,65 +401,6 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
 /* ---------------------------------------------------------- batched kernel */
 
 /*
- * Persistent form for many datasets at once: a fixed grid of wavefronts
- * pulls (job, chain) items from a global counter (largest work first) and
- * runs each chain for all of its steps; the shape of each job selects the
- * register layout. Every wave leaves when the counter passes the last item.
- */
-__global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
-  __shared__ __align__(16) float lds_all[4][128];
-  const uint32_t wave = threadIdx.x >> 6;
-  float *lds = lds_all[wave];
-  for (;;) {
-    uint32_t item = 0u;
-    if (lane_id() == 0u) item = atomicAdd(a.counter, 1u);
-    item = rfl(item);
-    if (item >= a.n_items) break;
-    const uint32_t packed = a.items[item];
-    const uint32_t j = packed >> 16, c = packed & 0xffffu;
-    const BatchJob &J = a.jobs[j];
-    const uint64_t rows = 2ull * ((uint64_t)J.row_offset + (uint64_t)c * J.n_nodes);
-    const uint32_t gc = J.chain_offset + c;
-    ChainView v;
-    v.cells = a.cells + 4ull * J.cell_offset;
-    v.am = a.am + rows;
-    v.dm = a.dm + rows;
-    v.best_am = a.best_am + rows;
-    v.best_dm = a.best_dm + rows;
-    v.beta = a.beta + gc;
-    v.score = a.score + gc;
-    v.best_beta = a.best_beta + gc;
-    v.best_score = a.best_score + gc;
-    v.best_step = a.best_step + gc;
-    v.trace = nullptr;
-    v.seed = J.seed;
-    v.job = J.job_tag;
-    v.chain = c;
-    v.step_begin = J.step_begin;
-    v.step_end = J.step_end;
-    v.n_cells = J.n_cells;
-    v.n_nodes = J.n_nodes;
-    v.is_first_chain = c == 0u;
-    v.score_init = J.score_init != 0u;
-    v.L00 = J.L00;
-    v.L10 = J.L10;
-    v.gamma = J.gamma;
-    v.prior = J.prior;
-    v.probs = J.probs;
-    switch (J.shape) {
-#define XS(NS_, CS_, W_)                                                       \
-  case BBS_SHAPE_CODE(NS_, CS_, W_):                                           \
-    run_chain<NS_, CS_, W_, false>(v, lds);                                    \
-    break;
-      BBS_SHAPES(XS)
-#undef XS
-    default:
-      break;
-    }
-  }
-}
-
-/*
  * Batched form with one wavefront per (job, chain) item: the items of one
  * register layout (shape) are launched together, largest work first, so
  * the hardware dispatcher hands the next item to whichever SIMD frees up
@@ -465,7 +410,7 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchArgs a) {
 template <int NS, int CS, int W>
 __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t first,
                                                           uint32_t count) {
-  __shared__ __align__(16) float lds_all[4][64 * CS];
+  __shared__ WaveLds lds_all[4];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t k = blockIdx.x * (blockDim.x >> 6) + wave;
   if (k >= count) return;
@@ -503,40 +448,11 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
   run_chain<NS, CS, W, false>(v, lds_all[wave]);
 }
 
-/* Initial states of every (job, chain) of a batch: uniform random trees
- * from each job's own seed, as the plain form draws them. */
-__global__ __launch_bounds__(64) void batch_init_kernel(BatchArgs a) {
-  for (uint32_t item = blockIdx.x; item < a.n_items; item += gridDim.x) {
-    const uint32_t packed = a.items[item];
-    const uint32_t j = packed >> 16, c = packed & 0xffffu;
-    const BatchJob &J = a.jobs[j];
-    TreeGenArgs g;
-    const uint64_t rows = 2ull * ((uint64_t)J.row_offset + (uint64_t)c * J.n_nodes);
-    g.am = a.am + rows;
-    g.dm = a.dm + rows;
-    g.beta = a.beta + J.chain_offset + c;
-    g.beta_value = J.beta_mean;
-    g.seed = J.seed;
-    g.job = J.job_tag;
-    g.chain_offset = c;
-    g.purpose = BBS_PURPOSE_INIT_TREE;
-    g.n_trees = 1u;
-    g.n_nodes = J.n_nodes;
-    random_tree_wave(g, 0u);
-    if (lane_id() == 0u) {
-      a.best_step[J.chain_offset + c] = 0xffffffffu;
-      a.best_score[J.chain_offset + c] = __int_as_float(0xff800000);
-      a.best_beta[J.chain_offset + c] = __int_as_float(0xff800000);
-    }
-  }
-  if (blockIdx.x == 0u && lane_id() == 0u) *a.counter = 0u;
-}
-
 /* ------------------------------------------------------------ score kernel */
 
 template <int NS, int CS, int W, bool COUNTS>
 __global__ __launch_bounds__(64) void score_kernel(ScoreArgs a) {
-  __shared__ __align__(16) float lds[64 * CS];
+  __shared__ WaveLds lds;
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
   Cells<CS, W> cells;
@@ -565,13 +481,13 @@ __global__ __launch_bounds__(64) void score_kernel(ScoreArgs a) {
       L.L11 = a.L11;
       prior = a.prior_value;
     }
-    uint32_t arg01[CS], arg11[CS];
+    uint32_t arg01[2], arg11[2];
     float T = score_tree<NS, CS, W, COUNTS>(dm, cells, N, a.n_cells, L, lds,
                                             arg01, arg11);
     if (lane == 0u) a.ll[tree] = a.with_prior ? T + prior : T;
     if (COUNTS) {
 #pragma unroll
-      for (int j = 0; j < CS; j++) {
+      for (int j = 0; j < Slots<CS>::n; j++) {
         const uint32_t cell = lane + 64u * (uint32_t)j;
         if (cell < a.n_cells) {
           const uint32_t n01 = arg01[j], n11 = arg11[j];
@@ -864,9 +780,10 @@ hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1
 
 /* ------------------------------------------------------------ dispatch */
 
-void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w) {
+void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
+              bool paired) {
   *ns = (n_genes + 1u) <= 64u ? 1 : 2;
-  *cs = n_cells <= 64u ? 1 : 2;
+  *cs = n_cells <= 64u ? 1 : (paired && n_cells <= 96u ? 3 : 2);
   *w = (int)((n_genes + 31u) / 32u);
   if (*w < 1) *w = 1;
   if (*ns == 2 && *w < 2) *w = 2;
@@ -874,7 +791,7 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w) {
 
 hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream) {
   int ns, cs, w;
-  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w);
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
   dim3 grid(a.n_chains), block(64);
 #define X(NS_, CS_, W_)                                                        \
   if (ns == NS_ && cs == CS_ && w == W_) {                                     \
@@ -894,7 +811,7 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream) {
 hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32) {
   int ns, cs, w;
-  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w);
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, false);
   if (!counts && cells32 != nullptr) {
     const int cmax = a.n_cells <= 32u ? 32 : a.n_cells <= 64u ? 64 : a.n_cells <= 96u ? 96 : 128;
     const int wq = (int)((a.n_nodes - 1u + 31u) / 32u);
@@ -924,7 +841,7 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                          stream, a);                                           \
     return hipGetLastError();                                                  \
   }
-  BBS_SHAPES(X)
+  BBS_FULL_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
 }
@@ -944,22 +861,6 @@ hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first
   BBS_SHAPES(XB)
 #undef XB
   return hipErrorInvalidValue;
-}
-
-hipError_t launch_batch_init(const BatchArgs &a, hipStream_t stream) {
-  uint32_t blocks = a.n_items < 65536u ? a.n_items : 65536u;
-  if (blocks == 0u) blocks = 1u;
-  hipLaunchKernelGGL(batch_init_kernel, dim3(blocks), dim3(64), 0, stream, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_batch(const BatchArgs &a, uint32_t groups, uint32_t waves_per_group,
-                        hipStream_t stream) {
-  if (groups == 0u) groups = 1u;
-  if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(batch_kernel, dim3(groups), dim3(64u * waves_per_group), 0, stream,
-                     a);
-  return hipGetLastError();
 }
 
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream) {

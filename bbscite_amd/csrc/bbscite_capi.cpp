@@ -851,10 +851,8 @@ struct bbs_batch {
   bbs_batch_config cfg{};
   uint32_t n_cus = 0;
   uint64_t total_steps = 0;
-  DevBuf d_jobs, d_items, d_counter, cells, am, dm, best_am, best_dm, beta, score,
+  DevBuf d_jobs, d_items, cells, am, dm, best_am, best_dm, beta, score,
       best_beta, best_score, best_step;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   bool ran = false;
   std::string err;
 };
@@ -922,7 +920,7 @@ int fill_job(const bbs_job &in, bbs::BatchJob &J, std::string &why) {
   J.n_nodes = in.n_genes + 1u;
   J.n_cells = in.n_cells;
   int ns, cs, w;
-  bbs::shape_of(in.n_genes, in.n_cells, &ns, &cs, &w);
+  bbs::shape_of(in.n_genes, in.n_cells, &ns, &cs, &w, true);
   J.shape = (uint32_t)(ns * 100 + cs * 10 + w);
   J.n_chains = p.n_chains;
   J.step_begin = 0u;
@@ -955,7 +953,8 @@ int fill_job(const bbs_job &in, bbs::BatchJob &J, std::string &why) {
 /* Relative device cost of one step of a job (VALU work of the scorer). */
 double step_cost(const bbs::BatchJob &J) {
   const int cs = (J.shape / 10) % 10, w = J.shape % 10;
-  return (double)J.n_nodes * cs * (4.0 * w + 14.0) + 600.0;
+  const double slots = cs == 1 ? 1.0 : cs == 2 ? 2.0 : 1.5;
+  return (double)J.n_nodes * slots * (4.0 * w + 12.0) + 600.0;
 }
 
 void order_items(bbs_batch *b) {
@@ -996,7 +995,6 @@ bbs::BatchArgs batch_args(bbs_batch *b) {
   a.jobs = (const bbs::BatchJob *)b->d_jobs.p;
   a.items = (const uint32_t *)b->d_items.p;
   a.n_items = (uint32_t)b->items.size();
-  a.counter = (uint32_t *)b->d_counter.p;
   a.cells = (const uint64_t *)b->cells.p;
   a.am = (uint64_t *)b->am.p;
   a.dm = (uint64_t *)b->dm.p;
@@ -1030,9 +1028,7 @@ void bbs_batch_destroy(bbs_batch *b) {
   if (!b) return;
   if (b->device >= 0) (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  DevBuf *bufs[] = {&b->d_jobs,  &b->d_items,   &b->d_counter, &b->cells,
+  DevBuf *bufs[] = {&b->d_jobs,  &b->d_items,   &b->cells,
                     &b->am,      &b->dm,        &b->best_am,   &b->best_dm,
                     &b->beta,    &b->score,     &b->best_beta, &b->best_score,
                     &b->best_step};
@@ -1125,7 +1121,6 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   int rc;
   if ((rc = balloc(b, b->d_jobs, b->jobs.size() * sizeof(bbs::BatchJob))) ||
       (rc = balloc(b, b->d_items, b->items.size() * 4u)) ||
-      (rc = balloc(b, b->d_counter, 4u)) ||
       (rc = balloc(b, b->cells, cells.size() * 8u)) || (rc = balloc(b, b->am, rows_b)) ||
       (rc = balloc(b, b->dm, rows_b)) || (rc = balloc(b, b->best_am, rows_b)) ||
       (rc = balloc(b, b->best_dm, rows_b)) || (rc = balloc(b, b->beta, sc_b)) ||

@@ -152,6 +152,11 @@ CHAIN_CASES = [
     (64, 33, 4, 2, 150, 0.1, 0.25, 0.1),
     (94, 95, 5, 3, 150, 0.1, 0.3, 0.05),
     (127, 128, 6, 2, 100, 0.1, 0.4309, 0.1),
+    (40, 80, 7, 2, 150, 0.1, 0.25, 0.1),
+    (20, 96, 8, 2, 150, 0.0, 0.25, 0.1),
+    (64, 65, 9, 2, 120, 0.1, 0.3, 0.1),
+    (3, 70, 10, 3, 200, 0.1, 0.25, 0.1),
+    (2, 1, 11, 2, 100, 0.1, 0.25, 0.1),
 ]
 
 
