@@ -264,6 +264,39 @@ __device__ __forceinline__ void pair_counts(const uint32_t (&km)[W],
   }
 }
 
+/* ((0 + x[0]) + x[1]) + ... over 4 * NQ floats of LDS (NQ a multiple of 4),
+ * straight-line code: while one group of 16 is added the reads of the next
+ * are in flight.  The empty asm on the running sum pins each group's
+ * adds before the next group's reads, so the compiler neither issues every
+ * read up front (registers) nor each read next to its use (latency). */
+template <int NQ>
+__device__ __forceinline__ float sum_padded(const float *x) {
+  static_assert(NQ % 4 == 0, "groups of 16 floats");
+  constexpr int NG = NQ / 4;
+  const float4 *q = reinterpret_cast<const float4 *>(x);
+  float4 g[NG][4];
+  float T = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NG + 1; k++) {
+    if (k < NG) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) g[k < NG ? k : 0][i] = q[4 * (k < NG ? k : 0) + i];
+    }
+    if (k >= 1) {
+      const int a = k - 1;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        T = T + g[a][i].x;
+        T = T + g[a][i].y;
+        T = T + g[a][i].z;
+        T = T + g[a][i].w;
+      }
+      asm volatile("" : "+v"(T));
+    }
+  }
+  return T;
+}
+
 /*
  * TreeScorer::logscore_tree without the beta prior (src/TreeScorer.hpp:
  * 131-205): per cell the maximum over nodes (a NaN at node 0 sticks, later
@@ -316,8 +349,9 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
   /* the paired slot: lanes < 32 score node v, lanes >= 32 node v + 1 */
   auto half = [&](const uint4 &A, bool first) {
     uint32_t n01, n11;
-    pair_counts<W>(c.km[1], c.knm[1], A, n01, n11);
-    const float s = pair_score(n01, n11, c.fa[1], c.fb[1], L);
+    constexpr int H = CS == 3 ? 1 : 0;
+    pair_counts<W>(c.km[H], c.knm[H], A, n01, n11);
+    const float s = pair_score(n01, n11, c.fa[H], c.fb[H], L);
     if (first) {
       best[1] = s;
       nan0[1] = lane_id() < 32u && s != s;
@@ -326,46 +360,56 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
     }
   };
   const uint32_t hi_half = lane_id() >> 5; /* 0 or 1 */
-  if (CS == 3) {
-    const uint32_t last = n_nodes - 1u;
+  const uint32_t last = n_nodes - 1u;
+  auto clampn = [&](uint32_t v) { return v < last ? v : last; };
+  /* rows are read one iteration ahead: with one wave per SIMD nothing else
+   * hides the LDS latency */
+  if constexpr (CS == 3) {
     full(S.rows[0], true);
+    half(S.rows[clampn(hi_half)], true);
     if (n_nodes > 1u) full(S.rows[1], false);
-    half(S.rows[hi_half < last ? hi_half : last], true);
+    uint4 A0 = S.rows[clampn(2u)], A1 = S.rows[clampn(3u)], Ah = S.rows[clampn(2u + hi_half)];
     for (uint32_t v = 2; v < n_nodes; v += 2) {
-      const uint32_t vh = v + hi_half;
-      const uint4 Ah = S.rows[vh < last ? vh : last];
-      full(S.rows[v], false);
-      if (v + 1u < n_nodes) full(S.rows[v + 1u], false);
-      half(Ah, false);
+      const uint4 B0 = A0, B1 = A1, Bh = Ah;
+      A0 = S.rows[clampn(v + 2u)];
+      A1 = S.rows[clampn(v + 3u)];
+      Ah = S.rows[clampn(v + 2u + hi_half)];
+      full(B0, false);
+      half(Bh, false);
+      if (v + 1u < n_nodes) full(B1, false);
     }
   } else {
     full(S.rows[0], true);
-    for (uint32_t v = 1; v < n_nodes; v++) full(S.rows[v], false);
+    uint4 A1 = S.rows[clampn(1u)], A2 = S.rows[clampn(2u)];
+    for (uint32_t v = 1; v < n_nodes; v += 2) {
+      const uint4 B1 = A1, B2 = A2;
+      A1 = S.rows[clampn(v + 2u)];
+      A2 = S.rows[clampn(v + 3u)];
+      full(B1, false);
+      if (v + 1u < n_nodes) full(B2, false);
+    }
   }
+  /* Cells past n_cells store +0.0: the running sum starts at +0.0 and never
+   * becomes -0.0, so adding +0.0 leaves it bit-identical. */
   const float qnan = __int_as_float(0x7fc00000);
-  S.sums[lane_id()] = nan0[0] ? qnan : best[0];
-  if (SL > 1) S.sums[64u + lane_id()] = nan0[1] ? qnan : best[1];
-  wave_lds_sync();
-  float T = 0.0f;
-  const uint32_t n_lo = n_cells < 64u ? n_cells : 64u;
-  uint32_t cc = 0;
-  for (; cc + 4u <= n_lo; cc += 4u) {
-    const float4 q = *reinterpret_cast<const float4 *>(S.sums + cc);
-    T = T + q.x;
-    T = T + q.y;
-    T = T + q.z;
-    T = T + q.w;
-  }
-  for (; cc < n_lo; cc++) T = T + S.sums[cc];
-  if (CS == 3) {
+  const uint32_t lane = lane_id();
+  S.sums[lane] = lane < n_cells ? (nan0[0] ? qnan : best[0]) : 0.0f;
+  if constexpr (CS == 3) {
     /* cell 64 + i: the maximum over even nodes (lane i) and odd nodes (lane
      * 32 + i); a NaN from node 0 only comes from the even half */
-    for (; cc < n_cells; cc++) {
-      const float x = S.sums[cc], y = S.sums[cc + 32u];
-      T = T + (x != x ? x : vmax(x, y));
-    }
+    const float x = nan0[1] ? qnan : best[1];
+    const float y = __shfl_xor(x, 32);
+    if (lane < 32u) S.sums[64u + lane] = 64u + lane < n_cells ? (x != x ? x : vmax(x, y)) : 0.0f;
+  } else if (SL > 1) {
+    S.sums[64u + lane] = 64u + lane < n_cells ? (nan0[1] ? qnan : best[1]) : 0.0f;
+  }
+  wave_lds_sync();
+  float T;
+  if constexpr (CS == 1) {
+    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
+                                                                 : sum_padded<16>(S.sums);
   } else {
-    for (; cc < n_cells; cc++) T = T + S.sums[cc];
+    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
   }
   wave_lds_sync();
   return rfl_f(T);

@@ -44,6 +44,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def chain_kernel_name(genes, cells):
+    """Template instance the chain launch picks for this size (the layout
+    rule of shape_of in bbs_kernels.hip: node slots, cell slots with the
+    paired half slot for 65..96 cells, 32-bit gene words)."""
+    ns = 1 if genes + 1 <= 64 else 2
+    cs = 1 if cells <= 64 else (3 if cells <= 96 else 2)
+    w = max(1, (genes + 31) // 32)
+    if ns == 2:
+        w = max(w, 2)
+    return f"chain_kernel<{ns},{cs},{w}>"
+
+
 def make_context(n_chains, n_steps):
     m, _, _ = B.generate_dataset(2, G, C, 1e-4, 0.3, 0.15)
     mut, kno = B.encode_matrix(m)
@@ -263,7 +275,7 @@ def main():
             "peak_measured": peak_ops / 1e12,
             "traffic": None,
             "alg_int_ops_per_step": ALG_INT_OPS,
-            "kernel": "chain_kernel<2,2,3>",
+            "kernel": chain_kernel_name(G, C),
         },
         "kernel_steps_per_s": k_rate,
         "best_log_likelihood": best_score,
