@@ -92,14 +92,20 @@ def cpu_baseline(p, mut, kno, seconds_budget=20.0):
     }
 
 
-def scoring_bench(reps=5, peak=P_INT32_TOPS):
-    """2^20 uniform random trees (seed 3) against one 94 x 95 iid matrix."""
+def scoring_matrix():
+    """One 94 x 95 matrix of iid entries, P(1) = 0.3, P(3) = 0.1, else 0
+    (numpy generator, seed 3), encoded as per-cell rows."""
     rng = np.random.default_rng(3)
     u = rng.random((G, C))
     m = np.zeros((G, C), np.int8)
     m[u < 0.3] = 1
     m[(u >= 0.3) & (u < 0.4)] = 3
-    mut, kno = B.encode_matrix(m)
+    return B.encode_matrix(m)
+
+
+def scoring_bench(reps=5, peak=P_INT32_TOPS):
+    """2^20 uniform random trees (seed 3) against one 94 x 95 iid matrix."""
+    mut, kno = scoring_matrix()
     p = B.default_params(n_chains=1, n_steps=1)
     n_trees = 1 << 20
     with B.Context(p, mut, kno, G) as ctx:
@@ -113,6 +119,34 @@ def scoring_bench(reps=5, peak=P_INT32_TOPS):
         "roofline": {"bound": "valu-int32", "achieved": achieved, "peak": peak,
                      "unit": "TOP/s", "frac": achieved / peak},
         "counted_bits_per_s": trees_s * 4 * N ** 3,
+    }
+
+
+def cpu_scoring_baseline(seconds_budget=5.0):
+    """The oracle's CPUTreeScorer-structured scorer (node-outer, cell-inner,
+    popcounts over 64-bit words, TreeScorer semantics, no prior) over the
+    first trees of the same seed-3 stream, parallel over trees on the host
+    cores (bounded sample)."""
+    try:
+        from tests import _oracle as O
+    except Exception as e:  # pragma: no cover
+        log("cpu scoring baseline unavailable:", e)
+        return None
+    cores = min(16, len(os.sched_getaffinity(0)))
+    mut, kno = scoring_matrix()
+    p = B.default_params(n_chains=1, n_steps=1)
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    r1, _ = O.bench_score(s, B.random_trees(3, G, 256), p.beta_mean, 1)
+    n = int(min(1 << 20, max(cores * 256, r1 * cores * seconds_budget)))
+    rate, _ = O.bench_score(s, B.random_trees(3, G, n), p.beta_mean, cores)
+    return {
+        "value": rate,
+        "unit": "trees/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {n} trees of the seed-3 stream at 94x95 (1-core rate {r1:.0f} trees/s)",
+        "one_core_value": r1,
+        "counted_bits_per_s": rate * 4 * N ** 3,
     }
 
 
@@ -242,6 +276,8 @@ def main():
         log(f"[bench] batched {batched['batched_steps_per_s']:.3e} steps/s, plain "
             f"{batched['plain_steps_per_s']:.3e} steps/s, x{batched['speedup']:.2f}")
     cpu = None if a.no_cpu else cpu_baseline(p, mut, kno)
+    if score is not None and not a.no_cpu:
+        score["cpu_baseline"] = cpu_scoring_baseline()
 
     out = {
         "metric": "chain steps/s at 94x95, batched persistent vs plain launch; p99 job latency",
