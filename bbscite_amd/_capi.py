@@ -119,6 +119,7 @@ _SIGS = {
     "bbs_create": (C.c_int, [C.POINTER(Params), _u32, _u32, _P, _P, C.POINTER(_P)]),
     "bbs_set_initial_states": (C.c_int, [_P, _P, _P, _P, _P]),
     "bbs_set_trace": (C.c_int, [_P, _u32]),
+    "bbs_set_scoring_mode": (C.c_int, [_P, C.c_int]),
     "bbs_run": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "bbs_begin": (C.c_int, [_P]),
     "bbs_advance": (C.c_int, [_P, _u32, C.POINTER(C.c_float)]),
@@ -345,6 +346,11 @@ class Context:
 
     def set_trace(self, n):
         self._c(lib().bbs_set_trace(self.h, n))
+
+    def set_scoring_mode(self, mode):
+        """"auto", "full" or "incremental" (results are identical)."""
+        code = {"auto": 0, "full": 1, "incremental": 2}[mode]
+        self._c(lib().bbs_set_scoring_mode(self.h, code))
 
     def run(self):
         ms = C.c_float()

@@ -218,7 +218,10 @@ __device__ __forceinline__ void load_cells(const uint64_t *cells,
  * sum. */
 struct WaveLds {
   uint4 rows[128];
-  float sums[192];
+  float sums[128];
+  /* incremental scoring: 0 for a node whose pair scores in the table are
+   * current, -inf for a changed node or a padding node (n_nodes .. 131) */
+  float pen[132];
 };
 
 /* The four log-probabilities of src/TreeScorer.hpp:99-106,136-137. */
@@ -297,21 +300,16 @@ __device__ __forceinline__ float sum_padded(const float *x) {
   return T;
 }
 
-/*
- * TreeScorer::logscore_tree without the beta prior (src/TreeScorer.hpp:
- * 131-205): per cell the maximum over nodes (a NaN at node 0 sticks, later
- * NaNs are ignored, as std::max(old, new) does), then the sum over cells in
- * order. Lanes hold cells; the tree's rows are copied to S.rows and read as
- * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
- * or 2 only), n01/n11 of the first maximal node are returned per slot.
- */
-template <int NS, int CS, int W, bool COUNTS>
-__device__ __forceinline__ float
-score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
-           uint32_t n_cells, const LogProbs &L, WaveLds &S,
-           uint32_t (&arg01)[2], uint32_t (&arg11)[2]) {
-  static_assert(!COUNTS || CS != 3, "counts need full cell slots");
-  constexpr int SL = Slots<CS>::n;
+/* Width in floats of one node's row of the pair-score table: the cells one
+ * wave scores (64, 96 with the paired half slot, or 128). */
+template <int CS>
+struct TableWidth {
+  static constexpr uint32_t n = CS == 1 ? 64u : CS == 3 ? 96u : 128u;
+};
+
+/* The tree's ancestor rows (dm, one per node) into LDS for broadcast reads. */
+template <int NS>
+__device__ __forceinline__ void store_rows(const uint64_t (&dm)[NS][NS], WaveLds &S) {
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     const uint64_t lo = dm[k][0], hi = NS > 1 ? dm[k][NS - 1] : 0ull;
@@ -319,15 +317,78 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
         make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
   }
   wave_lds_sync();
-  float best[2];
-  bool nan0[2] = {false, false};
-  /* one node for the full slots (node 0 seeds the maximum) */
-  auto full = [&](const uint4 &A, bool first) {
+}
+
+/* S.pen for a proposal: -inf for the nodes whose ancestor row changed
+ * (this lane's `diff` per node slot) and for padding nodes, 0 otherwise. */
+template <int NS>
+__device__ __forceinline__ void store_pen(const bool (&diff)[NS], uint32_t n_nodes, WaveLds &S) {
+  const float ninf = __int_as_float(0xff800000);
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane_id() + 64u * (uint32_t)k;
+    S.pen[x] = (diff[k] || x >= n_nodes) ? ninf : 0.0f;
+  }
+  if (lane_id() < 4u) S.pen[128u + lane_id()] = ninf;
+}
+
+/* Per-cell maxima -> ordered cell sum (src/TreeScorer.hpp:198-205).
+ * best[j]/nan0[j] of the full slots, and for the paired slot the maxima of
+ * the two lane halves. Cells past n_cells store +0.0: the running sum starts
+ * at +0.0 and never becomes -0.0, so adding +0.0 leaves it bit-identical. */
+template <int CS>
+__device__ __forceinline__ float finish_sum(const float (&best)[2], const bool (&nan0)[2],
+                                            uint32_t n_cells, WaveLds &S) {
+  constexpr int SL = Slots<CS>::n;
+  const float qnan = __int_as_float(0x7fc00000);
+  const uint32_t lane = lane_id();
+  S.sums[lane] = lane < n_cells ? (nan0[0] ? qnan : best[0]) : 0.0f;
+  if constexpr (CS == 3) {
+    /* cell 64 + i: the maximum over the nodes of lane i and of lane 32 + i;
+     * node 0 is always scored by the low half, so only its flag can make the
+     * cell NaN (a NaN of a later node never wins, as in the reference) */
+    const float x = nan0[1] ? qnan : best[1];
+    const float y = __shfl_xor(x, 32);
+    const float m = x != x ? x : vmax(x, y);
+    if (lane < 32u) S.sums[64u + lane] = 64u + lane < n_cells ? m : 0.0f;
+  } else if (SL > 1) {
+    S.sums[64u + lane] = 64u + lane < n_cells ? (nan0[1] ? qnan : best[1]) : 0.0f;
+  }
+  wave_lds_sync();
+  float T;
+  if constexpr (CS == 1) {
+    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
+                                                                 : sum_padded<16>(S.sums);
+  } else {
+    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
+  }
+  wave_lds_sync();
+  return rfl_f(T);
+}
+
+/* Node loop of the full scorer over the rows in S.rows: per cell the
+ * maximum over nodes (node 0 seeds it, later nodes replace it only when
+ * strictly greater, src/TreeScorer.hpp:185-194). With a table, every pair
+ * score is also stored at tab[node * width + cell]. With one wave per SIMD
+ * nothing else hides the LDS latency, so rows are read one node pair ahead. */
+template <int NS, int CS, int W, bool COUNTS>
+__device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_nodes,
+                                            const LogProbs &L, WaveLds &S, float *tab,
+                                            float (&best)[2], bool (&nan0)[2],
+                                            uint32_t (&arg01)[2], uint32_t (&arg11)[2]) {
+  static_assert(!COUNTS || CS != 3, "counts need full cell slots");
+  constexpr int SL = Slots<CS>::n;
+  constexpr uint32_t TW = TableWidth<CS>::n;
+  const uint32_t lane = lane_id();
+  nan0[0] = nan0[1] = false;
+  /* one node for the full slots */
+  auto full = [&](const uint4 &A, uint32_t node, bool first) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
       uint32_t n01, n11;
       pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
       const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
+      if (tab) tab[node * TW + 64u * (uint32_t)j + lane] = s;
       if (first) {
         best[j] = s;
         nan0[j] = s != s;
@@ -347,72 +408,227 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
     }
   };
   /* the paired slot: lanes < 32 score node v, lanes >= 32 node v + 1 */
-  auto half = [&](const uint4 &A, bool first) {
+  auto half = [&](const uint4 &A, uint32_t node, bool first) {
     uint32_t n01, n11;
     constexpr int H = CS == 3 ? 1 : 0;
     pair_counts<W>(c.km[H], c.knm[H], A, n01, n11);
     const float s = pair_score(n01, n11, c.fa[H], c.fb[H], L);
+    if (tab) tab[node * TW + 64u + (lane & 31u)] = s;
     if (first) {
       best[1] = s;
-      nan0[1] = lane_id() < 32u && s != s;
+      nan0[1] = lane < 32u && s != s;
     } else {
       best[1] = vmax(best[1], s);
     }
   };
-  const uint32_t hi_half = lane_id() >> 5; /* 0 or 1 */
+  const uint32_t hi_half = lane >> 5; /* 0 or 1 */
   const uint32_t last = n_nodes - 1u;
   auto clampn = [&](uint32_t v) { return v < last ? v : last; };
-  /* rows are read one iteration ahead: with one wave per SIMD nothing else
-   * hides the LDS latency */
   if constexpr (CS == 3) {
-    full(S.rows[0], true);
-    half(S.rows[clampn(hi_half)], true);
-    if (n_nodes > 1u) full(S.rows[1], false);
+    full(S.rows[0], 0u, true);
+    half(S.rows[clampn(hi_half)], clampn(hi_half), true);
+    if (n_nodes > 1u) full(S.rows[1], 1u, false);
     uint4 A0 = S.rows[clampn(2u)], A1 = S.rows[clampn(3u)], Ah = S.rows[clampn(2u + hi_half)];
     for (uint32_t v = 2; v < n_nodes; v += 2) {
       const uint4 B0 = A0, B1 = A1, Bh = Ah;
       A0 = S.rows[clampn(v + 2u)];
       A1 = S.rows[clampn(v + 3u)];
       Ah = S.rows[clampn(v + 2u + hi_half)];
-      full(B0, false);
-      half(Bh, false);
-      if (v + 1u < n_nodes) full(B1, false);
+      full(B0, v, false);
+      half(Bh, clampn(v + hi_half), false);
+      if (v + 1u < n_nodes) full(B1, v + 1u, false);
     }
   } else {
-    full(S.rows[0], true);
+    full(S.rows[0], 0u, true);
     uint4 A1 = S.rows[clampn(1u)], A2 = S.rows[clampn(2u)];
     for (uint32_t v = 1; v < n_nodes; v += 2) {
       const uint4 B1 = A1, B2 = A2;
       A1 = S.rows[clampn(v + 2u)];
       A2 = S.rows[clampn(v + 3u)];
-      full(B1, false);
-      if (v + 1u < n_nodes) full(B2, false);
+      full(B1, v, false);
+      if (v + 1u < n_nodes) full(B2, v + 1u, false);
     }
   }
-  /* Cells past n_cells store +0.0: the running sum starts at +0.0 and never
-   * becomes -0.0, so adding +0.0 leaves it bit-identical. */
-  const float qnan = __int_as_float(0x7fc00000);
+}
+
+/*
+ * TreeScorer::logscore_tree without the beta prior (src/TreeScorer.hpp:
+ * 131-205): per cell the maximum over nodes (a NaN at node 0 sticks, later
+ * NaNs are ignored, as std::max(old, new) does), then the sum over cells in
+ * order. Lanes hold cells; the tree's rows are copied to S.rows and read as
+ * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
+ * or 2 only), n01/n11 of the first maximal node are returned per slot.
+ */
+template <int NS, int CS, int W, bool COUNTS>
+__device__ __forceinline__ float
+score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
+           uint32_t n_cells, const LogProbs &L, WaveLds &S,
+           uint32_t (&arg01)[2], uint32_t (&arg11)[2], float *tab = nullptr) {
+  store_rows<NS>(dm, S);
+  float best[2];
+  bool nan0[2];
+  node_maxima<NS, CS, W, COUNTS>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
+  return finish_sum<CS>(best, nan0, n_cells, S);
+}
+
+/* Lowest set bit of a node mask held uniformly by the wave (bit v = node v),
+ * cleared from the mask; the mask must not be empty. */
+template <int NS>
+__device__ __forceinline__ uint32_t pop_node(uint64_t (&m)[NS]) {
+  if (NS == 1 || m[0] != 0ull) {
+    const uint32_t v = (uint32_t)__ffsll((unsigned long long)m[0]) - 1u;
+    m[0] &= m[0] - 1ull;
+    return v;
+  }
+  const uint32_t v = 64u + (uint32_t)__ffsll((unsigned long long)m[NS - 1]) - 1u;
+  m[NS - 1] &= m[NS - 1] - 1ull;
+  return v;
+}
+
+/* Pair scores of the nodes in `changed` (rows in S.rows), two nodes at a
+ * time so that the paired half slot serves both. WRITE stores them in the
+ * table; otherwise their per-cell maximum goes to R (node 0 flags NaN). */
+template <int NS, int CS, int W, bool WRITE>
+__device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
+                                              const LogProbs &L, WaveLds &S, float *tab,
+                                              float (&R)[2], bool (&nan0)[2]) {
+  constexpr int SL = Slots<CS>::n;
+  constexpr uint32_t TW = TableWidth<CS>::n;
   const uint32_t lane = lane_id();
-  S.sums[lane] = lane < n_cells ? (nan0[0] ? qnan : best[0]) : 0.0f;
-  if constexpr (CS == 3) {
-    /* cell 64 + i: the maximum over even nodes (lane i) and odd nodes (lane
-     * 32 + i); a NaN from node 0 only comes from the even half */
-    const float x = nan0[1] ? qnan : best[1];
-    const float y = __shfl_xor(x, 32);
-    if (lane < 32u) S.sums[64u + lane] = 64u + lane < n_cells ? (x != x ? x : vmax(x, y)) : 0.0f;
-  } else if (SL > 1) {
-    S.sums[64u + lane] = 64u + lane < n_cells ? (nan0[1] ? qnan : best[1]) : 0.0f;
+  const uint32_t hi_half = lane >> 5;
+  uint64_t m[NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) m[k] = changed[k];
+  bool more = false;
+#pragma unroll
+  for (int k = 0; k < NS; k++) more |= m[k] != 0ull;
+  while (more) {
+    const uint32_t a = pop_node<NS>(m);
+    bool more_b = false;
+#pragma unroll
+    for (int k = 0; k < NS; k++) more_b |= m[k] != 0ull;
+    const uint32_t b = more_b ? pop_node<NS>(m) : a;
+    const uint4 Aa = S.rows[a], Ab = S.rows[b];
+    const uint32_t h = hi_half ? b : a;
+    const uint4 Ah = S.rows[h];
+#pragma unroll
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      uint32_t n01, n11;
+      pair_counts<W>(c.km[j], c.knm[j], Aa, n01, n11);
+      const float sa = pair_score(n01, n11, c.fa[j], c.fb[j], L);
+      pair_counts<W>(c.km[j], c.knm[j], Ab, n01, n11);
+      const float sb = pair_score(n01, n11, c.fa[j], c.fb[j], L);
+      if (WRITE) {
+        tab[a * TW + 64u * (uint32_t)j + lane] = sa;
+        tab[b * TW + 64u * (uint32_t)j + lane] = sb;
+      } else {
+        R[j] = vmax(vmax(R[j], sa), sb);
+        if (a == 0u) nan0[j] = sa != sa;
+      }
+    }
+    if constexpr (CS == 3) {
+      uint32_t n01, n11;
+      pair_counts<W>(c.km[1], c.knm[1], Ah, n01, n11);
+      const float sh = pair_score(n01, n11, c.fa[1], c.fb[1], L);
+      if (WRITE) {
+        tab[h * TW + 64u + (lane & 31u)] = sh;
+      } else {
+        R[1] = vmax(R[1], sh);
+        if (a == 0u && hi_half == 0u) nan0[1] = sh != sh;
+      }
+    }
+    more = false;
+#pragma unroll
+    for (int k = 0; k < NS; k++) more |= m[k] != 0ull;
   }
-  wave_lds_sync();
-  float T;
-  if constexpr (CS == 1) {
-    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
-                                                                 : sum_padded<16>(S.sums);
-  } else {
-    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
+}
+
+/* Per-cell maximum over the stored scores of the nodes not in `changed`:
+ * the score plus S.pen[node] (0 or -inf) for every node, two nodes per
+ * iteration, branch-free (the paired half slot: lanes < 32 even nodes,
+ * lanes >= 32 odd nodes). The table has one padding row after the last
+ * node, masked by its -inf. Adding +0.0 turns a -0.0 score (a cell without
+ * known entries, where every node scores -0.0) into +0.0, which the ordered
+ * cell sum does not see. */
+template <int NS, int CS>
+__device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_nodes,
+                                                const float *tab, const WaveLds &S,
+                                                float (&U)[2], bool (&nan0)[2]) {
+  constexpr int SL = Slots<CS>::n;
+  constexpr uint32_t TW = TableWidth<CS>::n;
+  const uint32_t lane = lane_id();
+  const uint32_t hi_half = lane >> 5;
+  if (!node0_changed) {
+#pragma unroll
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      const float x = tab[64u * (uint32_t)j + lane];
+      nan0[j] = x != x;
+    }
+    if constexpr (CS == 3) {
+      const float x = tab[64u + (lane & 31u)];
+      if (hi_half == 0u) nan0[1] = x != x;
+    }
   }
-  wave_lds_sync();
-  return rfl_f(T);
+  /* Reads one node pair ahead through two alternating buffers; the empty
+   * asm with a memory clobber keeps the compiler from sinking the reads of
+   * the next pair below the maxima of the current one. Row indices past the
+   * last node are clamped to real rows or the padding row; their pen
+   * entries are -inf. */
+  const uint32_t lastb = n_nodes - 1u; /* rows lastb, lastb + 1 exist */
+  const float *pf = tab + lane;
+  const float *pfh = tab + 64u + (lane & 31u) + hi_half * TW;
+  struct Pair {
+    float x[2][2], xh;
+    float2 q;
+  };
+  auto rd = [&](uint32_t v, Pair &b) {
+    const uint32_t rb = (v < lastb ? v : lastb) * TW;
+#pragma unroll
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      b.x[j][0] = pf[rb + 64u * (uint32_t)j];
+      b.x[j][1] = pf[rb + TW + 64u * (uint32_t)j];
+    }
+    if constexpr (CS == 3) b.xh = pfh[rb];
+    b.q = *reinterpret_cast<const float2 *>(S.pen + v);
+    asm volatile("" ::: "memory");
+  };
+  auto use = [&](const Pair &b) {
+#pragma unroll
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      U[j] = vmax(U[j], b.x[j][0] + b.q.x);
+      U[j] = vmax(U[j], b.x[j][1] + b.q.y);
+    }
+    if constexpr (CS == 3) U[1] = vmax(U[1], b.xh + (hi_half ? b.q.y : b.q.x));
+  };
+  Pair A, B;
+  rd(0u, A);
+  for (uint32_t v = 0; v < n_nodes; v += 4) {
+    rd(v + 2u, B);
+    use(A);
+    if (v + 2u >= n_nodes) break;
+    rd(v + 4u, A);
+    use(B);
+  }
+}
+
+/* The tree score without the prior from the pair-score table of the
+ * current tree, for a proposal whose ancestor rows (in S.rows) differ from
+ * the current ones only at the nodes in `changed` (S.pen marks them): new
+ * scores for those nodes, stored scores for the rest. The per-cell maximum over nodes does
+ * not depend on the order the nodes are visited in (the first-wins rule
+ * only matters for a NaN at node 0, kept by the flags), so the result is
+ * bit-identical to the full scorer. */
+template <int NS, int CS, int W>
+__device__ __forceinline__ float score_changed(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
+                                               uint32_t n_nodes, uint32_t n_cells,
+                                               const LogProbs &L, WaveLds &S, const float *tab) {
+  const float ninf = __int_as_float(0xff800000);
+  float R[2] = {ninf, ninf}, U[2] = {ninf, ninf};
+  bool nan0[2] = {false, false};
+  changed_nodes<NS, CS, W, false>(changed, c, L, S, nullptr, R, nan0);
+  unchanged_nodes<NS, CS>((changed[0] & 1ull) != 0ull, n_nodes, tab, S, U, nan0);
+  float best[2] = {vmax(R[0], U[0]), vmax(R[1], U[1])};
+  return finish_sum<CS>(best, nan0, n_cells, S);
 }
 
 /* Constants of the beta prior (src/TreeScorer.hpp:108-111,122-126). */

@@ -166,9 +166,14 @@ struct ChainView {
 };
 
 /* Steps [step_begin, step_end) of one chain: the WorkKernel loop body of
- * src/Application.hpp:412-487 with the state resident in VGPRs. */
-template <int NS, int CS, int W, bool TRACE>
-__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
+ * src/Application.hpp:412-487 with the state resident in VGPRs. With INCR
+ * the pair scores of
+ * the current tree are kept in `tab` (n_nodes x TableWidth<CS> floats of
+ * LDS): a tree move rescores only the nodes whose ancestor row changed and
+ * takes the other nodes' scores from the table, which is updated when the
+ * move is accepted; beta moves rescore every pair. */
+template <int NS, int CS, int W, bool TRACE, bool INCR>
+__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, float *tab) {
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
 
@@ -195,12 +200,13 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
   float cur_prior = logscore_beta(a.prior, cur_beta);
   uint32_t unused01[2], unused11[2];
   float cur_score;
-  if (a.score_init)
-    cur_score = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds,
-                                             unused01, unused11) +
-                cur_prior;
-  else
+  if (a.score_init || INCR) {
+    const float t = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds, unused01,
+                                                 unused11, INCR ? tab : nullptr);
+    cur_score = a.score_init ? t + cur_prior : *a.score;
+  } else {
     cur_score = *a.score;
+  }
 
   float best_score = *a.best_score;
   float best_beta = *a.best_beta;
@@ -277,10 +283,25 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
       Lp.L11 = bbs_logf_of_double(1.0 - (double)nb);
       prop_prior = logscore_beta(a.prior, nb);
     }
-    const float prop_score =
-        score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
-                                     unused11) +
-        prop_prior;
+    uint64_t changed[NS];
+    float prop_raw;
+    if (INCR && m.type != BBS_MOVE_CHANGE_BETA) {
+      bool diff[NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        diff[k] = false;
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) diff[k] |= pdm[k][wd] != dm[k][wd];
+        changed[k] = ballot(diff[k]);
+      }
+      store_pen<NS>(diff, N, lds);
+      store_rows<NS>(pdm, lds);
+      prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
+    } else {
+      prop_raw = score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
+                                              unused11);
+    }
+    const float prop_score = prop_raw + prop_prior;
 
     float corr = 1.0f;
     if (m.type == BBS_MOVE_SWAP_SUBTREES && m.related)
@@ -317,6 +338,16 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
         }
     }
     if (accept) {
+      if (INCR) {
+        /* the table follows the accepted tree (its rows are in lds.rows) */
+        float R[2], best[2];
+        bool n0[2];
+        if (m.type == BBS_MOVE_CHANGE_BETA)
+          node_maxima<NS, CS, W, false>(cells, N, Lp, lds, tab, best, n0, unused01, unused11);
+        else
+          changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
+        wave_lds_sync();
+      }
       cur_score = prop_score;
       cur_beta = prop_beta;
       L = Lp;
@@ -362,9 +393,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds) {
 /* ------------------------------------------------------------ plain kernel */
 
 /* One dataset per launch, one wavefront per chain (the drop-in form). */
-template <int NS, int CS, int W, bool TRACE>
+template <int NS, int CS, int W, bool TRACE, bool INCR>
 __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   __shared__ WaveLds lds;
+  extern __shared__ float table[]; /* (n_nodes + 1) x TableWidth<CS> with INCR */
   const uint32_t chain = blockIdx.x;
   if (chain >= a.n_chains) return;
   const uint64_t rows = 2ull * chain * a.n_nodes;
@@ -395,7 +427,7 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.gamma = a.gamma;
   v.prior = a.prior;
   v.probs = a.probs;
-  run_chain<NS, CS, W, TRACE>(v, lds);
+  run_chain<NS, CS, W, TRACE, INCR>(v, lds, INCR ? table : nullptr);
 }
 
 /* ---------------------------------------------------------- batched kernel */
@@ -445,7 +477,7 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
   v.gamma = J.gamma;
   v.prior = J.prior;
   v.probs = J.probs;
-  run_chain<NS, CS, W, false>(v, lds_all[wave]);
+  run_chain<NS, CS, W, false, false>(v, lds_all[wave], nullptr);
 }
 
 /* ------------------------------------------------------------ score kernel */
@@ -789,20 +821,57 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
   if (*ns == 2 && *w < 2) *w = 2;
 }
 
-hipError_t launch_chain(const ChainArgs &a, bool trace, hipStream_t stream) {
+/* Layouts with an incremental-scoring chain kernel: the pair-score table of
+ * the 128-cell layout with two node slots would exceed 64 KB of LDS. */
+template <int NS, int CS>
+constexpr bool has_table() {
+  return CS != 2 || NS == 1;
+}
+
+uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
+  int ns, cs, w;
+  shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
+  if (cs == 2 && ns == 2) return 0u;
+  const uint32_t width = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
+  const uint32_t bytes = (n_nodes + 1u) * width * 4u; /* + one padding row */
+  return bytes + (uint32_t)sizeof(WaveLds) <= 65536u ? bytes : 0u;
+}
+
+uint32_t chain_static_lds() { return (uint32_t)sizeof(WaveLds); }
+
+template <int NS, int CS, int W>
+static hipError_t launch_chain_shape(const ChainArgs &a, bool trace, uint32_t table_bytes,
+                                     hipStream_t stream) {
+  const dim3 grid(a.n_chains), block(64);
+  if constexpr (has_table<NS, CS>()) {
+    if (table_bytes) {
+      if (trace)
+        hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, true>), grid, block, table_bytes,
+                           stream, a);
+      else
+        hipLaunchKernelGGL((chain_kernel<NS, CS, W, false, true>), grid, block, table_bytes,
+                           stream, a);
+      return hipGetLastError();
+    }
+  } else {
+    if (table_bytes) return hipErrorInvalidValue;
+  }
+  if (trace)
+    hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, false>), grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL((chain_kernel<NS, CS, W, false, false>), grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
+                        hipStream_t stream) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
-  dim3 grid(a.n_chains), block(64);
+  if (table_bytes && table_bytes < (a.n_nodes + 1u) * (cs == 1 ? 64u : cs == 3 ? 96u : 128u) * 4u)
+    return hipErrorInvalidValue;
 #define X(NS_, CS_, W_)                                                        \
-  if (ns == NS_ && cs == CS_ && w == W_) {                                     \
-    if (trace)                                                                 \
-      hipLaunchKernelGGL((chain_kernel<NS_, CS_, W_, true>), grid, block, 0,   \
-                         stream, a);                                           \
-    else                                                                       \
-      hipLaunchKernelGGL((chain_kernel<NS_, CS_, W_, false>), grid, block, 0,  \
-                         stream, a);                                           \
-    return hipGetLastError();                                                  \
-  }
+  if (ns == NS_ && cs == CS_ && w == W_)                                       \
+    return launch_chain_shape<NS_, CS_, W_>(a, trace, table_bytes, stream);
   BBS_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

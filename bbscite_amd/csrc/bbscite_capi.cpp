@@ -52,6 +52,8 @@ struct bbs_ctx {
   DevBuf cells, cells32, am, dm, beta, score, best_am, best_dm, best_beta,
       best_score, best_step, trace;
   uint32_t n_traced = 0;
+  int scoring_mode = BBS_SCORING_AUTO;
+  uint32_t n_cus = 0, lds_per_cu = 0;
   bool states_set = false;   /* initial states present on the device */
   bool score_known = false;  /* initial scores present on the device  */
   bool begun = false;        /* bests reset, steps counted from 0     */
@@ -293,12 +295,18 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
     if (is_gfx950(d)) dev = d;
   if (dev < 0) return fail(nullptr, BBS_ERR_NODEV, "no gfx950 device found");
 
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(nullptr, BBS_ERR_NODEV, "hipGetDeviceProperties failed");
+
   bbs_ctx *ctx = new bbs_ctx();
   ctx->params = p;
   ctx->n_cells = n_cells;
   ctx->n_genes = n_genes;
   ctx->n_nodes = n_genes + 1u;
   ctx->device = dev;
+  ctx->n_cus = (uint32_t)prop.multiProcessorCount;
+  ctx->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
 
   /* TreeScorer constructor (src/TreeScorer.hpp:92-117) */
   ctx->L00 = (float)std::log(1.0 - (double)p.alpha_mean);
@@ -480,6 +488,34 @@ int bbs_begin(bbs_ctx *ctx) {
 }
 
 /* Advance every chain by n steps (continuing the current run). */
+/* LDS bytes of the pair-score table for this context's launches, or 0 for
+ * full rescoring. Automatic mode takes the table when it leaves every CU
+ * room for as many chain waves as the launch can give it (at most one per
+ * SIMD), so the table never costs concurrency. */
+static uint32_t chain_table(const bbs_ctx *ctx) {
+  if (ctx->scoring_mode == BBS_SCORING_FULL) return 0u;
+  const uint32_t t = bbs::chain_table_bytes(ctx->n_nodes, ctx->n_cells);
+  if (t == 0u || ctx->scoring_mode == BBS_SCORING_INCREMENTAL) return t;
+  const uint32_t per_wave = t + bbs::chain_static_lds();
+  const uint32_t cus = ctx->n_cus ? ctx->n_cus : 1u;
+  uint32_t want = (ctx->params.n_chains + cus - 1u) / cus;
+  if (want > 4u) want = 4u;
+  if (want == 0u) want = 1u;
+  return (uint64_t)per_wave * want <= ctx->lds_per_cu ? t : 0u;
+}
+
+int bbs_set_scoring_mode(bbs_ctx *ctx, int mode) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (mode != BBS_SCORING_AUTO && mode != BBS_SCORING_FULL && mode != BBS_SCORING_INCREMENTAL)
+    return fail(ctx, BBS_ERR_ARG, "unknown scoring mode %d", mode);
+  if (mode == BBS_SCORING_INCREMENTAL && bbs::chain_table_bytes(ctx->n_nodes, ctx->n_cells) == 0u)
+    return fail(ctx, BBS_ERR_SIZE,
+                "no incremental scorer for %u genes x %u cells (table over 64 KB of LDS)",
+                ctx->n_genes, ctx->n_cells);
+  ctx->scoring_mode = mode;
+  return BBS_OK;
+}
+
 int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
   if (!ctx->begun) {
@@ -518,9 +554,10 @@ int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   a.gamma = ctx->params.gamma;
   a.prior = ctx->prior;
   a.probs = ctx->probs;
+  const uint32_t table = chain_table(ctx);
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (a.n_chains > 0u)
-    HIP_TRY(ctx, bbs::launch_chain(a, ctx->n_traced > 0u, ctx->stream));
+    HIP_TRY(ctx, bbs::launch_chain(a, ctx->n_traced > 0u, table, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   float ms = 0.f;

@@ -107,6 +107,18 @@ int bbs_set_initial_states(bbs_ctx *ctx, const uint64_t *am, const uint64_t *dm,
 /* Record a bbs_step_trace for every step of chains [0, n_traced_chains). */
 int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
 
+/* How the chain kernel scores proposals; results are bit-identical in every
+ * mode (no reference counterpart: the reference rescores every pair,
+ * src/TreeScorer.hpp:131-210). FULL rescores all (node, cell) pairs each
+ * step; INCREMENTAL keeps the current tree's pair scores in LDS and
+ * rescores only the nodes whose ancestor set a move changes; AUTO (the
+ * default) uses INCREMENTAL when its table fits without costing concurrent
+ * chains. BBS_ERR_SIZE if INCREMENTAL is impossible for this size. */
+#define BBS_SCORING_AUTO 0
+#define BBS_SCORING_FULL 1
+#define BBS_SCORING_INCREMENTAL 2
+int bbs_set_scoring_mode(bbs_ctx *ctx, int mode);
+
 /* Replaces Application::run_simulation (src/Application.hpp:197-219): runs
  * all chains for n_steps steps and blocks; *makespan_ms (nullable) receives
  * the device time in milliseconds, measured with HIP events. */

@@ -160,12 +160,23 @@ CHAIN_CASES = [
 ]
 
 
+def _set_mode(ctx, mode):
+    try:
+        ctx.set_scoring_mode(mode)
+    except B.BbsError as e:
+        if mode == "incremental" and e.code == -2:
+            pytest.skip("no incremental scorer for this size (table over 64 KB of LDS)")
+        raise
+
+
+@pytest.mark.parametrize("mode", ["full", "incremental"])
 @pytest.mark.parametrize("case", CHAIN_CASES, ids=lambda c: f"G{c[0]}C{c[1]}")
-def test_chain_trace_replay(case):
+def test_chain_trace_replay(case, mode):
     G, C, seed, R, S, pb, bm, bsd = case
-    say(f"[chain] G={G} C={C} chains={R} steps={S}")
+    say(f"[chain] G={G} C={C} chains={R} steps={S} {mode}")
     p, mut, kno = _chain_case(G, C, seed, R, S, pb, beta_mean=bm, beta_sd=bsd)
     with B.Context(p, mut, kno, G) as ctx:
+        _set_mode(ctx, mode)
         ctx.set_trace(R)
         ctx.run()
         trace = ctx.trace(R)
@@ -196,6 +207,31 @@ def test_chain_trace_replay(case):
     # every move type occurs
     types = set(np.unique(trace["move_type"]).tolist())
     assert {1, 2, 3} <= types and (pb == 0 or 0 in types)
+
+
+@pytest.mark.parametrize("G,C", [(94, 95), (63, 64), (127, 96), (30, 128), (5, 7)])
+def test_incremental_equals_full_over_long_runs(G, C):
+    """Thousands of steps with many accepted moves of every kind: the
+    pair-score table stays exact, so both scoring modes give identical
+    chains (scores, best steps, final trees, betas)."""
+    say(f"[incremental] G={G} C={C}")
+    p, mut, kno = _chain_case(G, C, 12, 16, 4000, 0.1, beta_mean=0.3, beta_sd=0.05)
+    out = {}
+    for mode in ("full", "incremental"):
+        with B.Context(p, mut, kno, G) as ctx:
+            _set_mode(ctx, mode)
+            ctx.begin()
+            for n in (1000, 3000):
+                ctx.advance(n)
+            out[mode] = (ctx.chain_results(), ctx.chain_states(), ctx.best())
+    (r0, s0, b0), (r1, s1, b1) = out["full"], out["incremental"]
+    for x, y in zip(r0, r1):
+        assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                              _bits(y) if y.dtype == np.float32 else y)
+    for x, y in zip(s0, s1):
+        assert np.array_equal(x, y)
+    assert np.array_equal(b0[0], b1[0]) and np.array_equal(b0[1], b1[1])
+    assert _bits(b0[3]) == _bits(b1[3])
 
 
 def test_advance_windows_equal_one_run():
