@@ -217,11 +217,17 @@ __device__ __forceinline__ void load_cells(const uint64_t *cells,
  * read as broadcasts by the scorer) and the per-cell maxima for the ordered
  * sum. */
 struct WaveLds {
+  uint4 *rows;  /* [n_nodes] ancestor rows of the tree being scored       */
+  float *sums;  /* [cells of the layout] per-cell maxima                 */
+  float *pen;   /* [TableView::pen_len] incremental scoring: 0 for a node
+                 * whose stored pair scores are current, -inf for a changed
+                 * or padding node                                        */
+};
+/* LDS of one wave for the full scorer */
+struct WaveLdsStore {
   uint4 rows[128];
   float sums[128];
-  /* incremental scoring: 0 for a node whose pair scores in the table are
-   * current, -inf for a changed node or a padding node (n_nodes .. 131) */
-  float pen[132];
+  __device__ WaveLds view() { return WaveLds{rows, sums, nullptr}; }
 };
 
 /* The four log-probabilities of src/TreeScorer.hpp:99-106,136-137. */
@@ -300,21 +306,35 @@ __device__ __forceinline__ float sum_padded(const float *x) {
   return T;
 }
 
-/* Width in floats of one node's row of the pair-score table: the cells one
- * wave scores (64, 96 with the paired half slot, or 128). */
+/* The pair-score table of the current tree in LDS, node-major per cell:
+ * the scores of cell c (the cell a lane scores in one slot: 64 cells, 96
+ * with the paired half slot, or 128) over all nodes form one row of
+ * `stride` floats. stride = 8 ceil(n_nodes / 8) + 4 is 4 times an odd
+ * number, so the 16-byte reads of 16 consecutive lanes (one row each) hit
+ * distinct banks; a row also covers the padding nodes the 4-node groups of
+ * the scan read. */
+struct TableView {
+  float *t;
+  uint32_t stride;
+  __device__ static uint32_t stride_for(uint32_t n_nodes) { return ((n_nodes + 7u) & ~7u) + 4u; }
+  __device__ static uint32_t pen_len(uint32_t n_nodes) { return stride_for(n_nodes) + 4u; }
+};
 template <int CS>
-struct TableWidth {
+struct TableRows {
   static constexpr uint32_t n = CS == 1 ? 64u : CS == 3 ? 96u : 128u;
 };
 
 /* The tree's ancestor rows (dm, one per node) into LDS for broadcast reads. */
 template <int NS>
-__device__ __forceinline__ void store_rows(const uint64_t (&dm)[NS][NS], WaveLds &S) {
+__device__ __forceinline__ void store_rows(const uint64_t (&dm)[NS][NS], uint32_t n_nodes,
+                                           WaveLds &S) {
 #pragma unroll
   for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane_id() + 64u * (uint32_t)k;
     const uint64_t lo = dm[k][0], hi = NS > 1 ? dm[k][NS - 1] : 0ull;
-    S.rows[lane_id() + 64u * (uint32_t)k] =
-        make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    if (x < n_nodes)
+      S.rows[x] =
+          make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
   }
   wave_lds_sync();
 }
@@ -324,12 +344,16 @@ __device__ __forceinline__ void store_rows(const uint64_t (&dm)[NS][NS], WaveLds
 template <int NS>
 __device__ __forceinline__ void store_pen(const bool (&diff)[NS], uint32_t n_nodes, WaveLds &S) {
   const float ninf = __int_as_float(0xff800000);
+  const uint32_t len = TableView::pen_len(n_nodes);
 #pragma unroll
-  for (int k = 0; k < NS; k++) {
+  for (int k = 0; k < 3; k++) {
     const uint32_t x = lane_id() + 64u * (uint32_t)k;
-    S.pen[x] = (diff[k] || x >= n_nodes) ? ninf : 0.0f;
+    bool d = x >= n_nodes;
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+      if (j == k) d = d || diff[j];
+    if (x < len) S.pen[x] = d ? ninf : 0.0f;
   }
-  if (lane_id() < 4u) S.pen[128u + lane_id()] = ninf;
 }
 
 /* Per-cell maxima -> ordered cell sum (src/TreeScorer.hpp:198-205).
@@ -368,18 +392,19 @@ __device__ __forceinline__ float finish_sum(const float (&best)[2], const bool (
 
 /* Node loop of the full scorer over the rows in S.rows: per cell the
  * maximum over nodes (node 0 seeds it, later nodes replace it only when
- * strictly greater, src/TreeScorer.hpp:185-194). With a table, every pair
- * score is also stored at tab[node * width + cell]. With one wave per SIMD
- * nothing else hides the LDS latency, so rows are read one node pair ahead. */
+ * strictly greater, src/TreeScorer.hpp:185-194). With a table (tab.t not
+ * null), every pair score is also stored. With one wave per SIMD nothing
+ * else hides the LDS latency, so rows are read one node pair ahead. */
 template <int NS, int CS, int W, bool COUNTS>
 __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_nodes,
-                                            const LogProbs &L, WaveLds &S, float *tab,
+                                            const LogProbs &L, WaveLds &S, TableView tab,
                                             float (&best)[2], bool (&nan0)[2],
                                             uint32_t (&arg01)[2], uint32_t (&arg11)[2]) {
   static_assert(!COUNTS || CS != 3, "counts need full cell slots");
   constexpr int SL = Slots<CS>::n;
-  constexpr uint32_t TW = TableWidth<CS>::n;
   const uint32_t lane = lane_id();
+  float *const trow0 = tab.t ? tab.t + lane * tab.stride : nullptr;
+  float *const trowh = tab.t ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
   nan0[0] = nan0[1] = false;
   /* one node for the full slots */
   auto full = [&](const uint4 &A, uint32_t node, bool first) {
@@ -388,7 +413,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
       uint32_t n01, n11;
       pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
       const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      if (tab) tab[node * TW + 64u * (uint32_t)j + lane] = s;
+      if (trow0) trow0[64u * (uint32_t)j * tab.stride + node] = s;
       if (first) {
         best[j] = s;
         nan0[j] = s != s;
@@ -413,7 +438,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
     constexpr int H = CS == 3 ? 1 : 0;
     pair_counts<W>(c.km[H], c.knm[H], A, n01, n11);
     const float s = pair_score(n01, n11, c.fa[H], c.fb[H], L);
-    if (tab) tab[node * TW + 64u + (lane & 31u)] = s;
+    if (trowh) trowh[node] = s;
     if (first) {
       best[1] = s;
       nan0[1] = lane < 32u && s != s;
@@ -463,8 +488,8 @@ template <int NS, int CS, int W, bool COUNTS>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
            uint32_t n_cells, const LogProbs &L, WaveLds &S,
-           uint32_t (&arg01)[2], uint32_t (&arg11)[2], float *tab = nullptr) {
-  store_rows<NS>(dm, S);
+           uint32_t (&arg01)[2], uint32_t (&arg11)[2], TableView tab = TableView{nullptr, 0u}) {
+  store_rows<NS>(dm, n_nodes, S);
   float best[2];
   bool nan0[2];
   node_maxima<NS, CS, W, COUNTS>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
@@ -490,12 +515,13 @@ __device__ __forceinline__ uint32_t pop_node(uint64_t (&m)[NS]) {
  * table; otherwise their per-cell maximum goes to R (node 0 flags NaN). */
 template <int NS, int CS, int W, bool WRITE>
 __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
-                                              const LogProbs &L, WaveLds &S, float *tab,
+                                              const LogProbs &L, WaveLds &S, TableView tab,
                                               float (&R)[2], bool (&nan0)[2]) {
   constexpr int SL = Slots<CS>::n;
-  constexpr uint32_t TW = TableWidth<CS>::n;
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
+  float *const trow0 = WRITE ? tab.t + lane * tab.stride : nullptr;
+  float *const trowh = WRITE ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
   uint64_t m[NS];
 #pragma unroll
   for (int k = 0; k < NS; k++) m[k] = changed[k];
@@ -519,8 +545,8 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
       pair_counts<W>(c.km[j], c.knm[j], Ab, n01, n11);
       const float sb = pair_score(n01, n11, c.fa[j], c.fb[j], L);
       if (WRITE) {
-        tab[a * TW + 64u * (uint32_t)j + lane] = sa;
-        tab[b * TW + 64u * (uint32_t)j + lane] = sb;
+        trow0[64u * (uint32_t)j * tab.stride + a] = sa;
+        trow0[64u * (uint32_t)j * tab.stride + b] = sb;
       } else {
         R[j] = vmax(vmax(R[j], sa), sb);
         if (a == 0u) nan0[j] = sa != sa;
@@ -531,7 +557,7 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
       pair_counts<W>(c.km[1], c.knm[1], Ah, n01, n11);
       const float sh = pair_score(n01, n11, c.fa[1], c.fb[1], L);
       if (WRITE) {
-        tab[h * TW + 64u + (lane & 31u)] = sh;
+        trowh[h] = sh;
       } else {
         R[1] = vmax(R[1], sh);
         if (a == 0u && hi_half == 0u) nan0[1] = sh != sh;
@@ -544,69 +570,79 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
 }
 
 /* Per-cell maximum over the stored scores of the nodes not in `changed`:
- * the score plus S.pen[node] (0 or -inf) for every node, two nodes per
- * iteration, branch-free (the paired half slot: lanes < 32 even nodes,
- * lanes >= 32 odd nodes). The table has one padding row after the last
- * node, masked by its -inf. Adding +0.0 turns a -0.0 score (a cell without
- * known entries, where every node scores -0.0) into +0.0, which the ordered
- * cell sum does not see. */
+ * score + S.pen[node] (0 or -inf) for every node, branch-free, four nodes
+ * per 16-byte read of this lane's table row. Full slots scan nodes
+ * [0, 4 ceil(n/4)); in the paired half slot lanes < 32 scan the first
+ * 4 ceil(n/8) nodes and lanes >= 32 the next as many. Padding nodes have
+ * pen -inf. Adding +0.0 turns a -0.0 score (a cell without known entries,
+ * where every node scores -0.0) into +0.0, which the ordered cell sum does
+ * not see. The next group's reads are issued before the current group's
+ * maxima; the empty asm with a memory clobber keeps them there. */
 template <int NS, int CS>
 __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_nodes,
-                                                const float *tab, const WaveLds &S,
+                                                TableView tab, const WaveLds &S,
                                                 float (&U)[2], bool (&nan0)[2]) {
   constexpr int SL = Slots<CS>::n;
-  constexpr uint32_t TW = TableWidth<CS>::n;
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
+  const uint32_t gh = (n_nodes + 7u) >> 3; /* 4-node groups per half */
+  const float *const r0 = tab.t + lane * tab.stride;
+  const float *const rh =
+      tab.t + (64u + (lane & 31u)) * tab.stride + (hi_half ? 4u * gh : 0u);
+  const float *const qh = S.pen + (hi_half ? 4u * gh : 0u);
   if (!node0_changed) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
-      const float x = tab[64u * (uint32_t)j + lane];
+      const float x = r0[64u * (uint32_t)j * tab.stride];
       nan0[j] = x != x;
     }
     if constexpr (CS == 3) {
-      const float x = tab[64u + (lane & 31u)];
+      const float x = tab.t[(64u + (lane & 31u)) * tab.stride];
       if (hi_half == 0u) nan0[1] = x != x;
     }
   }
-  /* Reads one node pair ahead through two alternating buffers; the empty
-   * asm with a memory clobber keeps the compiler from sinking the reads of
-   * the next pair below the maxima of the current one. Row indices past the
-   * last node are clamped to real rows or the padding row; their pen
-   * entries are -inf. */
-  const uint32_t lastb = n_nodes - 1u; /* rows lastb, lastb + 1 exist */
-  const float *pf = tab + lane;
-  const float *pfh = tab + 64u + (lane & 31u) + hi_half * TW;
-  struct Pair {
-    float x[2][2], xh;
-    float2 q;
+  /* one step = 8 nodes of the full slots (two groups) and one group of
+   * each half of the paired slot */
+  struct Grp {
+    float4 x[2][2], q[2], xh, qhv;
   };
-  auto rd = [&](uint32_t v, Pair &b) {
-    const uint32_t rb = (v < lastb ? v : lastb) * TW;
+  auto rd = [&](uint32_t g, Grp &b) { /* g: first 4-node group, even */
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
-      b.x[j][0] = pf[rb + 64u * (uint32_t)j];
-      b.x[j][1] = pf[rb + TW + 64u * (uint32_t)j];
+      b.x[j][0] = *reinterpret_cast<const float4 *>(r0 + 64u * (uint32_t)j * tab.stride + 4u * g);
+      b.x[j][1] =
+          *reinterpret_cast<const float4 *>(r0 + 64u * (uint32_t)j * tab.stride + 4u * g + 4u);
     }
-    if constexpr (CS == 3) b.xh = pfh[rb];
-    b.q = *reinterpret_cast<const float2 *>(S.pen + v);
+    b.q[0] = *reinterpret_cast<const float4 *>(S.pen + 4u * g);
+    b.q[1] = *reinterpret_cast<const float4 *>(S.pen + 4u * g + 4u);
+    if constexpr (CS == 3) {
+      b.xh = *reinterpret_cast<const float4 *>(rh + 2u * g);
+      b.qhv = *reinterpret_cast<const float4 *>(qh + 2u * g);
+    }
     asm volatile("" ::: "memory");
   };
-  auto use = [&](const Pair &b) {
+  auto mx4 = [&](float &u, const float4 &x, const float4 &q) {
+    u = vmax(u, x.x + q.x);
+    u = vmax(u, x.y + q.y);
+    u = vmax(u, x.z + q.z);
+    u = vmax(u, x.w + q.w);
+  };
+  auto use = [&](const Grp &b) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
-      U[j] = vmax(U[j], b.x[j][0] + b.q.x);
-      U[j] = vmax(U[j], b.x[j][1] + b.q.y);
+      mx4(U[j], b.x[j][0], b.q[0]);
+      mx4(U[j], b.x[j][1], b.q[1]);
     }
-    if constexpr (CS == 3) U[1] = vmax(U[1], b.xh + (hi_half ? b.q.y : b.q.x));
+    if constexpr (CS == 3) mx4(U[1], b.xh, b.qhv);
   };
-  Pair A, B;
+  const uint32_t g4 = (n_nodes + 3u) >> 2; /* 4-node groups of the full slots */
+  Grp A, B;
   rd(0u, A);
-  for (uint32_t v = 0; v < n_nodes; v += 4) {
-    rd(v + 2u, B);
+  for (uint32_t g = 0; g < g4; g += 4) {
+    if (g + 2u < g4) rd(g + 2u, B);
     use(A);
-    if (v + 2u >= n_nodes) break;
-    rd(v + 4u, A);
+    if (g + 2u >= g4) break;
+    if (g + 4u < g4) rd(g + 4u, A);
     use(B);
   }
 }
@@ -621,11 +657,11 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
 template <int NS, int CS, int W>
 __device__ __forceinline__ float score_changed(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
                                                uint32_t n_nodes, uint32_t n_cells,
-                                               const LogProbs &L, WaveLds &S, const float *tab) {
+                                               const LogProbs &L, WaveLds &S, TableView tab) {
   const float ninf = __int_as_float(0xff800000);
   float R[2] = {ninf, ninf}, U[2] = {ninf, ninf};
   bool nan0[2] = {false, false};
-  changed_nodes<NS, CS, W, false>(changed, c, L, S, nullptr, R, nan0);
+  changed_nodes<NS, CS, W, false>(changed, c, L, S, tab, R, nan0);
   unchanged_nodes<NS, CS>((changed[0] & 1ull) != 0ull, n_nodes, tab, S, U, nan0);
   float best[2] = {vmax(R[0], U[0]), vmax(R[1], U[1])};
   return finish_sum<CS>(best, nan0, n_cells, S);

@@ -88,13 +88,14 @@ struct BatchArgs {
  * half slot for 65..96 cells, used when paired is true), W gene words. */
 void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
               bool paired);
-/* table_bytes: 0 for the full scorer every step, else the LDS of the
- * pair-score table (chain_table_bytes) for incremental scoring. */
+/* table_bytes: 0 for the full scorer every step, else the dynamic LDS of an
+ * incremental chain wave (chain_table_bytes). */
 hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
                         hipStream_t stream);
-/* Bytes of the chain's pair-score table, 0 if the layout has none. */
+/* LDS bytes of an incremental chain wave (pair-score table, tree rows,
+ * sums, node penalties), 0 if the layout has none within 64 KB. */
 uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells);
-/* Static LDS of one chain wave. */
+/* Static LDS of an incremental chain wave besides chain_table_bytes. */
 uint32_t chain_static_lds();
 /* cells32: [128][10] words for the lane-per-tree kernel (used when counts
  * are not requested), or null to use the lane-per-cell kernel. */

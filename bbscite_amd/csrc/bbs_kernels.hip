@@ -167,13 +167,12 @@ struct ChainView {
 
 /* Steps [step_begin, step_end) of one chain: the WorkKernel loop body of
  * src/Application.hpp:412-487 with the state resident in VGPRs. With INCR
- * the pair scores of
- * the current tree are kept in `tab` (n_nodes x TableWidth<CS> floats of
- * LDS): a tree move rescores only the nodes whose ancestor row changed and
- * takes the other nodes' scores from the table, which is updated when the
- * move is accepted; beta moves rescore every pair. */
+ * the pair scores of the current tree are kept in the LDS table `tab`: a
+ * tree move rescores only the nodes whose ancestor row changed and takes
+ * the other nodes' scores from the table, which is updated when the move is
+ * accepted; beta moves rescore every pair. */
 template <int NS, int CS, int W, bool TRACE, bool INCR>
-__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, float *tab) {
+__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, TableView tab) {
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
 
@@ -202,7 +201,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, floa
   float cur_score;
   if (a.score_init || INCR) {
     const float t = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds, unused01,
-                                                 unused11, INCR ? tab : nullptr);
+                                                 unused11, INCR ? tab : TableView{nullptr, 0u});
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
     cur_score = *a.score;
@@ -295,7 +294,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, floa
         changed[k] = ballot(diff[k]);
       }
       store_pen<NS>(diff, N, lds);
-      store_rows<NS>(pdm, lds);
+      store_rows<NS>(pdm, N, lds);
       prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
     } else {
       prop_raw = score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
@@ -393,10 +392,25 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, floa
 /* ------------------------------------------------------------ plain kernel */
 
 /* One dataset per launch, one wavefront per chain (the drop-in form). */
+/* Dynamic LDS of an incremental chain wave, in floats: the pair-score
+ * table, then the tree rows, the per-cell sums and the node penalties. */
+struct ChainLds {
+  uint32_t table, rows, sums, pen, total;
+  __host__ __device__ static ChainLds of(uint32_t n_nodes, int cs) {
+    const uint32_t stride = ((n_nodes + 7u) & ~7u) + 4u;
+    ChainLds l;
+    l.table = 0u;
+    l.rows = (cs == 1 ? 64u : cs == 3 ? 96u : 128u) * stride;
+    l.sums = l.rows + 4u * n_nodes;
+    l.pen = l.sums + (cs == 1 ? 64u : cs == 3 ? 96u : 128u);
+    l.total = l.pen + stride + 4u;
+    return l;
+  }
+};
+
 template <int NS, int CS, int W, bool TRACE, bool INCR>
 __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
-  __shared__ WaveLds lds;
-  extern __shared__ float table[]; /* (n_nodes + 1) x TableWidth<CS> with INCR */
+  extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
   const uint32_t chain = blockIdx.x;
   if (chain >= a.n_chains) return;
   const uint64_t rows = 2ull * chain * a.n_nodes;
@@ -427,7 +441,16 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.gamma = a.gamma;
   v.prior = a.prior;
   v.probs = a.probs;
-  run_chain<NS, CS, W, TRACE, INCR>(v, lds, INCR ? table : nullptr);
+  if constexpr (INCR) {
+    const ChainLds l = ChainLds::of(a.n_nodes, CS);
+    WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
+    run_chain<NS, CS, W, TRACE, true>(
+        v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes)});
+  } else {
+    __shared__ WaveLdsStore store;
+    WaveLds lds = store.view();
+    run_chain<NS, CS, W, TRACE, false>(v, lds, TableView{nullptr, 0u});
+  }
 }
 
 /* ---------------------------------------------------------- batched kernel */
@@ -442,7 +465,7 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
 template <int NS, int CS, int W>
 __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t first,
                                                           uint32_t count) {
-  __shared__ WaveLds lds_all[4];
+  __shared__ WaveLdsStore lds_all[4];
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t k = blockIdx.x * (blockDim.x >> 6) + wave;
   if (k >= count) return;
@@ -477,14 +500,16 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
   v.gamma = J.gamma;
   v.prior = J.prior;
   v.probs = J.probs;
-  run_chain<NS, CS, W, false, false>(v, lds_all[wave], nullptr);
+  WaveLds lds = lds_all[wave].view();
+  run_chain<NS, CS, W, false, false>(v, lds, TableView{nullptr, 0u});
 }
 
 /* ------------------------------------------------------------ score kernel */
 
 template <int NS, int CS, int W, bool COUNTS>
 __global__ __launch_bounds__(64) void score_kernel(ScoreArgs a) {
-  __shared__ WaveLds lds;
+  __shared__ WaveLdsStore store;
+  WaveLds lds = store.view();
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
   Cells<CS, W> cells;
@@ -832,12 +857,11 @@ uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
   int ns, cs, w;
   shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
   if (cs == 2 && ns == 2) return 0u;
-  const uint32_t width = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
-  const uint32_t bytes = (n_nodes + 1u) * width * 4u; /* + one padding row */
-  return bytes + (uint32_t)sizeof(WaveLds) <= 65536u ? bytes : 0u;
+  const uint32_t bytes = 4u * ChainLds::of(n_nodes, cs).total;
+  return bytes <= 65536u ? bytes : 0u;
 }
 
-uint32_t chain_static_lds() { return (uint32_t)sizeof(WaveLds); }
+uint32_t chain_static_lds() { return 0u; }
 
 template <int NS, int CS, int W>
 static hipError_t launch_chain_shape(const ChainArgs &a, bool trace, uint32_t table_bytes,
@@ -867,7 +891,7 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
                         hipStream_t stream) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
-  if (table_bytes && table_bytes < (a.n_nodes + 1u) * (cs == 1 ? 64u : cs == 3 ? 96u : 128u) * 4u)
+  if (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, cs).total)
     return hipErrorInvalidValue;
 #define X(NS_, CS_, W_)                                                        \
   if (ns == NS_ && cs == CS_ && w == W_)                                       \
