@@ -106,6 +106,7 @@ class BatchConfig(C.Structure):
         ("n_streams", C.c_uint32),
         ("order", C.c_uint32),
         ("persistent", C.c_uint32),
+        ("scoring", C.c_uint32),
     ]
 
 
@@ -464,9 +465,12 @@ def _bcheck(rc, b=None):
         raise BbsError(rc, msg.decode() if msg else "")
 
 
-def batch_config(waves_per_group=None, n_streams=None, order=None):
+def batch_config(waves_per_group=None, n_streams=None, order=None, scoring=None):
+    """scoring: None/"auto" (incremental where the tables fit) or "full"."""
     c = BatchConfig()
     lib().bbs_batch_config_default(C.byref(c))
+    if scoring is not None:
+        c.scoring = {"auto": 0, "full": 1}[scoring]
     if waves_per_group is not None:
         c.waves_per_group = waves_per_group
     if n_streams is not None:

@@ -102,10 +102,15 @@ uint32_t chain_static_lds();
 hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
-/* items [first, first + count) all have the given shape code */
+/* items [first, first + count) all have the given shape code; wave_floats
+ * 0 for the full scorer, else the LDS floats of each wave's incremental
+ * slab (batch_wave_floats of the largest tree in the range) */
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
-                              uint32_t count, uint32_t waves_per_group,
+                              uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
                               hipStream_t stream);
+/* LDS floats of an incremental chain wave for trees of up to max_nodes
+ * nodes in this layout, 0 if the layout has no incremental scorer */
+uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes);
 
 hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
                             uint32_t iters, hipStream_t stream);

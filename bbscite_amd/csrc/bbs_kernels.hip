@@ -462,10 +462,10 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
  * first. waves_per_group wavefronts share a workgroup and run independent
  * items (run_chain has no workgroup barrier).
  */
-template <int NS, int CS, int W>
+template <int NS, int CS, int W, bool INCR>
 __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t first,
-                                                          uint32_t count) {
-  __shared__ WaveLdsStore lds_all[4];
+                                                          uint32_t count, uint32_t wave_floats) {
+  extern __shared__ __align__(16) float dyn[]; /* INCR: one ChainLds slab per wave */
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t k = blockIdx.x * (blockDim.x >> 6) + wave;
   if (k >= count) return;
@@ -500,8 +500,17 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
   v.gamma = J.gamma;
   v.prior = J.prior;
   v.probs = J.probs;
-  WaveLds lds = lds_all[wave].view();
-  run_chain<NS, CS, W, false, false>(v, lds, TableView{nullptr, 0u});
+  if constexpr (INCR) {
+    float *slab = dyn + wave * wave_floats;
+    const ChainLds l = ChainLds::of(J.n_nodes, CS);
+    WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
+    run_chain<NS, CS, W, false, true>(v, lds,
+                                      TableView{slab + l.table, TableView::stride_for(J.n_nodes)});
+  } else {
+    __shared__ WaveLdsStore lds_all[4];
+    WaveLds lds = lds_all[wave].view();
+    run_chain<NS, CS, W, false, false>(v, lds, TableView{nullptr, 0u});
+  }
 }
 
 /* ------------------------------------------------------------ score kernel */
@@ -939,18 +948,39 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
   return hipErrorInvalidValue;
 }
 
+uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes) {
+  const int ns = (int)(shape / 100u), cs = (int)(shape / 10u % 10u);
+  if (cs == 2 && ns == 2) return 0u;
+  return ChainLds::of(max_nodes, cs).total;
+}
+
+template <int NS, int CS, int W>
+static hipError_t launch_items_shape(const BatchArgs &a, uint32_t first, uint32_t count,
+                                     uint32_t wpg, uint32_t wave_floats, hipStream_t stream) {
+  const uint32_t groups = (count + wpg - 1u) / wpg;
+  if constexpr (has_table<NS, CS>()) {
+    if (wave_floats) {
+      hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true>), dim3(groups), dim3(64u * wpg),
+                         4u * wave_floats * wpg, stream, a, first, count, wave_floats);
+      return hipGetLastError();
+    }
+  } else {
+    if (wave_floats) return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, false>), dim3(groups), dim3(64u * wpg), 0,
+                     stream, a, first, count, 0u);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
-                              uint32_t count, uint32_t waves_per_group,
+                              uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
                               hipStream_t stream) {
   if (count == 0u) return hipSuccess;
   if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
-  const uint32_t groups = (count + waves_per_group - 1u) / waves_per_group;
+  if (4ull * wave_floats * waves_per_group > 65536ull) return hipErrorInvalidValue;
 #define XB(NS_, CS_, W_)                                                       \
-  if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_)) {                      \
-    hipLaunchKernelGGL((batch_items_kernel<NS_, CS_, W_>), dim3(groups),       \
-                       dim3(64u * waves_per_group), 0, stream, a, first, count); \
-    return hipGetLastError();                                                  \
-  }
+  if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_))                        \
+    return launch_items_shape<NS_, CS_, W_>(a, first, count, waves_per_group, wave_floats, stream);
   BBS_SHAPES(XB)
 #undef XB
   return hipErrorInvalidValue;

@@ -882,6 +882,7 @@ struct bbs_batch {
   std::vector<uint32_t> items;
   struct Range {
     uint32_t shape, first, count;
+    uint32_t max_nodes; /* largest tree of the range (LDS of its slabs) */
     double work;
   };
   std::vector<Range> ranges; /* items of one shape, heaviest range first */
@@ -1012,9 +1013,10 @@ void order_items(bbs_batch *b) {
   });
   for (uint32_t idx = 0; idx < order.size();) {
     const uint32_t shape = b->jobs[order[idx]].shape;
-    bbs_batch::Range r{shape, (uint32_t)b->items.size(), 0u, 0.0};
+    bbs_batch::Range r{shape, (uint32_t)b->items.size(), 0u, 0u, 0.0};
     for (; idx < order.size() && b->jobs[order[idx]].shape == shape; idx++) {
       const uint32_t j = order[idx];
+      r.max_nodes = std::max(r.max_nodes, b->jobs[j].n_nodes);
       for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
       r.work += work(j) * b->jobs[j].n_chains;
     }
@@ -1055,6 +1057,7 @@ void bbs_batch_config_default(bbs_batch_config *cfg) {
   cfg->n_streams = 4u;
   cfg->order = 0u;
   cfg->persistent = 0u;
+  cfg->scoring = BBS_SCORING_AUTO;
 }
 
 const char *bbs_batch_last_error(const bbs_batch *b) {
@@ -1102,7 +1105,8 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   else
     bbs_batch_config_default(&b->cfg);
   if (b->cfg.waves_per_group < 1u || b->cfg.waves_per_group > 4u ||
-      b->cfg.n_streams < 1u || b->cfg.n_streams > 4u) {
+      b->cfg.n_streams < 1u || b->cfg.n_streams > 4u ||
+      (b->cfg.scoring != BBS_SCORING_AUTO && b->cfg.scoring != BBS_SCORING_FULL)) {
     bbs_batch_destroy(b);
     return bfail(nullptr, BBS_ERR_ARG, "invalid batch configuration");
   }
@@ -1182,6 +1186,14 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   return BBS_OK;
 }
 
+/* LDS floats of each wave's incremental slab for a range, or 0 for full
+ * rescoring: incremental wherever the range's workgroups fit in 64 KB. */
+static uint32_t range_wave_floats(const bbs_batch *b, const bbs_batch::Range &R) {
+  if (b->cfg.scoring == BBS_SCORING_FULL) return 0u;
+  const uint32_t f = bbs::batch_wave_floats(R.shape, R.max_nodes);
+  return 4ull * f * b->cfg.waves_per_group <= 65536ull ? f : 0u;
+}
+
 int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
   if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
   if (b->cfg.persistent)
@@ -1226,7 +1238,8 @@ int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
   for (size_t r = 0; r < b->ranges.size() && e == hipSuccess; r++) {
     const auto &R = b->ranges[r];
     hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
-    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, b->cfg.waves_per_group, st);
+    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, b->cfg.waves_per_group,
+                                range_wave_floats(b, R), st);
   }
   for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++) {
     e = hipEventRecord(b->side_done[i], b->side[i]);

@@ -225,6 +225,8 @@ typedef struct bbs_batch_config {
   uint32_t order;           /* 0: largest work first; 1: job order      */
   uint32_t persistent;      /* 1: persistent graph-captured kernel;
                                switched off, bbs_batch_run rejects it   */
+  uint32_t scoring;         /* BBS_SCORING_AUTO (incremental where the
+                               pair-score tables fit) or _FULL          */
 } bbs_batch_config;
 
 typedef struct bbs_batch bbs_batch;

@@ -382,12 +382,14 @@ def test_batch_persistent_mode_is_rejected():
         assert e.value.code == B.ERR_STATE
 
 
-def test_batched_items_equal_plain_runs():
-    """Every job of a mixed batch (one wave per item, per-layout launches)
-    gives exactly the plain bbs_run result, on every rerun."""
-    say("[batched items]")
+@pytest.mark.parametrize("scoring", ["auto", "full"])
+def test_batched_items_equal_plain_runs(scoring):
+    """Every job of a mixed batch (one wave per item, per-layout launches,
+    incremental or full scoring) gives exactly the plain bbs_run result, on
+    every rerun."""
+    say(f"[batched items] {scoring}")
     jobs = _mix(12, 40, 150)
-    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4)) as b:
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4, scoring=scoring)) as b:
         b.run()
         batched = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
         b.run()
@@ -409,8 +411,9 @@ def test_batched_items_configs_agree_and_autotune():
     say("[batched configs]")
     jobs = _mix(6, 77, 60)
     ref = None
-    for wpg, nst, order in ((1, 1, 0), (4, 2, 1), (2, 4, 0)):
-        with B.Batch(jobs, B.batch_config(wpg, nst, order)) as b:
+    for wpg, nst, order, sc in ((1, 1, 0, "auto"), (4, 2, 1, "auto"), (2, 4, 0, "full"),
+                                (4, 1, 0, "full")):
+        with B.Batch(jobs, B.batch_config(wpg, nst, order, scoring=sc)) as b:
             b.run()
             res = [b.chain_results(j)[2] for j in range(len(jobs))]
         if ref is None:
