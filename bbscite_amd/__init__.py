@@ -7,6 +7,8 @@ only binds it (ctypes) and builds it. There is no CPU fallback.
 from ._capi import (  # noqa: F401
     Context,
     Params,
+    Server,
+    server_config,
     BbsError,
     default_params,
     encode_matrix,

@@ -110,6 +110,44 @@ class BatchConfig(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    """bbs_server_config (include/bbscite.h)."""
+
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("round_cost", C.c_uint32),
+        ("max_resident_chains", C.c_uint32),
+        ("waves_per_group", C.c_uint32),
+        ("n_streams", C.c_uint32),
+    ]
+
+
+class JobResult(C.Structure):
+    """bbs_job_result (include/bbscite.h)."""
+
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("n_genes", C.c_uint32),
+        ("n_cells", C.c_uint32),
+        ("n_chains", C.c_uint32),
+        ("n_steps", C.c_uint32),
+        ("best_score", C.c_float),
+        ("best_beta", C.c_float),
+        ("best_chain", C.c_uint32),
+        ("best_step", C.c_uint32),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+        ("best_am", C.c_uint64 * 2 * MAX_NODES),
+        ("best_dm", C.c_uint64 * 2 * MAX_NODES),
+    ]
+
+    def rows(self, which="dm"):
+        n = self.n_genes + 1
+        a = np.ctypeslib.as_array(getattr(self, "best_" + which))
+        return a.reshape(MAX_NODES, 2)[:n].copy()
+
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -146,6 +184,16 @@ _SIGS = {
         C.c_int,
         [C.POINTER(Job), _u32, _u32, C.POINTER(BatchConfig), C.POINTER(C.c_float)],
     ),
+    "bbs_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "bbs_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "bbs_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "bbs_server_step": (C.c_int, [_P, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    "bbs_server_poll": (C.c_int, [_P, C.POINTER(JobResult), _u32, C.POINTER(_u32)]),
+    "bbs_server_now": (C.c_double, [_P]),
+    "bbs_server_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_uint64)]),
+    "bbs_server_last_error": (C.c_char_p, [_P]),
+    "bbs_server_destroy": (None, [_P]),
     "bbs_destroy": (None, [_P]),
     "bbs_parse_csv": (C.c_int, [C.c_char_p, _u64, _P, C.POINTER(_u32), C.POINTER(_u32)]),
     "bbs_encode_matrix": (C.c_int, [_P, _u32, _u32, _P, _P]),
@@ -540,3 +588,66 @@ class Batch:
         b, s = C.c_float(), C.c_float()
         _bcheck(lib().bbs_batch_get_best(self.h, job, _ptr(am), _ptr(dm), C.byref(b), C.byref(s)), self.h)
         return am, dm, b.value, s.value
+
+
+# ---------------------------------------------------------------- serving
+
+
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    lib().bbs_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """A continuous-batching server (bbs_server_*): submit jobs at any time,
+    step() runs one round, poll() collects finished jobs."""
+
+    def __init__(self, cfg: ServerConfig = None):
+        self.h = C.c_void_p()
+        self._check(lib().bbs_server_create(C.byref(cfg) if cfg is not None else None,
+                                            C.byref(self.h)), create=True)
+
+    def _check(self, rc, create=False):
+        if rc != OK:
+            msg = lib().bbs_server_last_error(None if create else self.h)
+            raise BbsError(rc, msg.decode() if msg else "")
+
+    def submit(self, params, is_mutated, is_known, n_genes, job_tag=0):
+        arr, keep = _jobs_array([(params, is_mutated, is_known, n_genes, job_tag)])
+        jid = C.c_uint64()
+        self._check(lib().bbs_server_submit(self.h, arr, C.byref(jid)))
+        del keep  # the server copied the data
+        return jid.value
+
+    def step(self):
+        r, p, d = _u32(), _u32(), _u32()
+        self._check(lib().bbs_server_step(self.h, C.byref(r), C.byref(p), C.byref(d)))
+        return r.value, p.value, d.value
+
+    def poll(self, max_results=256):
+        out = (JobResult * max_results)()
+        n = _u32()
+        self._check(lib().bbs_server_poll(self.h, out, max_results, C.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def now(self):
+        return lib().bbs_server_now(self.h)
+
+    def stats(self):
+        st, bs, rd = C.c_uint64(), C.c_double(), C.c_uint64()
+        self._check(lib().bbs_server_stats(self.h, C.byref(st), C.byref(bs), C.byref(rd)))
+        return st.value, bs.value, rd.value
+
+    def close(self):
+        if self.h:
+            lib().bbs_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -15,7 +15,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -1543,6 +1546,530 @@ int64_t bbs_tree_to_newick(const uint64_t *am, const uint64_t *dm,
   out.put("\n");
   out.finish();
   return (int64_t)out.len;
+}
+
+} /* extern "C" */
+
+/* ------------------------------------------------------------------ serving */
+
+namespace {
+
+/* First-fit allocator over [0, cap) elements of one device array. */
+struct Arena {
+  uint64_t cap = 0;
+  std::map<uint64_t, uint64_t> free_; /* offset -> length */
+  void init(uint64_t c) {
+    cap = c;
+    free_.clear();
+    if (c) free_[0] = c;
+  }
+  bool alloc(uint64_t n, uint64_t &off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        off = it->first;
+        const uint64_t len = it->second;
+        free_.erase(it);
+        if (len > n) free_[off + n] = len - n;
+        return true;
+      }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    if (n == 0) return;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct SrvJob {
+  uint64_t id = 0;
+  bbs::BatchJob J{};     /* offsets set at admission */
+  uint32_t n_genes = 0, n_steps = 0, steps_done = 0;
+  std::vector<uint64_t> cells; /* [n_cells][4] */
+  double cost = 0.0;     /* step_cost(J) */
+  uint64_t lds = 0;      /* LDS bytes of one chain wave */
+  double t_submit = 0.0, t_admit = 0.0;
+  bool started = false;
+};
+
+} // namespace
+
+struct bbs_server {
+  int device = -1;
+  bbs_server_config cfg{};
+  hipStream_t stream[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t done_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t n_cus = 0, lds_per_cu = 0;
+  DevBuf am, dm, best_am, best_dm, beta, score, best_beta, best_score, best_step, cells,
+      d_jobs, d_items;
+  Arena rows, chains, cellsA;
+  std::deque<SrvJob *> pending;
+  std::vector<SrvJob *> resident;
+  std::deque<bbs_job_result> done;
+  uint64_t next_id = 1;
+  uint64_t lds_cap = 0, lds_used = 0;
+  uint32_t chain_cap = 0, chains_used = 0;
+  uint64_t steps_total = 0, rounds = 0;
+  double busy_s = 0.0;
+  std::chrono::steady_clock::time_point t0;
+  std::string err;
+};
+
+namespace {
+
+int sfail(bbs_server *s, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (s)
+    s->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define SHIP_TRY(s, expr)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return sfail((s), BBS_ERR_HIP, "%s failed: %s", #expr,                   \
+                   hipGetErrorString(e_));                                     \
+  } while (0)
+
+double srv_now(const bbs_server *s) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+int srv_alloc(bbs_server *s, DevBuf &d, size_t bytes) {
+  d.release();
+  if (bytes == 0) return BBS_OK;
+  hipError_t e = hipMalloc(&d.p, bytes);
+  if (e != hipSuccess) {
+    d.p = nullptr;
+    return sfail(s, e == hipErrorOutOfMemory ? BBS_ERR_BUDGET : BBS_ERR_HIP,
+                 "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  d.bytes = bytes;
+  return BBS_OK;
+}
+
+/* Grows a staging buffer (job table, item list) to at least `bytes`. */
+int srv_reserve(bbs_server *s, DevBuf &d, size_t bytes) {
+  if (d.bytes >= bytes) return BBS_OK;
+  return srv_alloc(s, d, std::max(bytes, 2 * d.bytes));
+}
+
+void srv_free_job(bbs_server *s, SrvJob *j) {
+  const bbs::BatchJob &J = j->J;
+  s->rows.release(J.row_offset, (uint64_t)J.n_chains * J.n_nodes);
+  s->chains.release(J.chain_offset, J.n_chains);
+  s->cellsA.release(J.cell_offset, J.n_cells);
+  s->lds_used -= (uint64_t)J.n_chains * j->lds;
+  s->chains_used -= J.n_chains;
+  delete j;
+}
+
+/* Admit the head of the queue if memory and concurrent-wave capacity allow
+ * (an empty server admits any job, so none waits forever). */
+int srv_admit(bbs_server *s, bool &admitted) {
+  admitted = false;
+  SrvJob *j = s->pending.front();
+  bbs::BatchJob &J = j->J;
+  const bool empty = s->resident.empty();
+  if (!empty && (s->chains_used + J.n_chains > s->chain_cap ||
+                 s->lds_used + (uint64_t)J.n_chains * j->lds > s->lds_cap))
+    return BBS_OK;
+  uint64_t r = 0, c = 0, x = 0;
+  const uint64_t n_rows = (uint64_t)J.n_chains * J.n_nodes;
+  if (!s->rows.alloc(n_rows, r)) return BBS_OK;
+  if (!s->chains.alloc(J.n_chains, c)) {
+    s->rows.release(r, n_rows);
+    return BBS_OK;
+  }
+  if (!s->cellsA.alloc(J.n_cells, x)) {
+    s->rows.release(r, n_rows);
+    s->chains.release(c, J.n_chains);
+    return BBS_OK;
+  }
+  J.row_offset = r;
+  J.chain_offset = (uint32_t)c;
+  J.cell_offset = (uint32_t)x;
+  s->lds_used += (uint64_t)J.n_chains * j->lds;
+  s->chains_used += J.n_chains;
+  SHIP_TRY(s, hipMemcpyAsync((uint64_t *)s->cells.p + 4ull * x, j->cells.data(),
+                             j->cells.size() * 8u, hipMemcpyHostToDevice, s->stream[0]));
+  j->t_admit = srv_now(s);
+  s->pending.pop_front();
+  s->resident.push_back(j);
+  admitted = true;
+  return BBS_OK;
+}
+
+/* The finished job's result: the chain-major first maximum of its chains
+ * (src/Application.hpp:456), as bbs_get_best. */
+int srv_retire(bbs_server *s, SrvJob *j, double t_done) {
+  const bbs::BatchJob &J = j->J;
+  std::vector<float> sc(J.n_chains), bb(J.n_chains);
+  std::vector<uint32_t> st(J.n_chains);
+  SHIP_TRY(s, hipMemcpy(sc.data(), (float *)s->best_score.p + J.chain_offset,
+                        J.n_chains * 4u, hipMemcpyDeviceToHost));
+  SHIP_TRY(s, hipMemcpy(st.data(), (uint32_t *)s->best_step.p + J.chain_offset,
+                        J.n_chains * 4u, hipMemcpyDeviceToHost));
+  SHIP_TRY(s, hipMemcpy(bb.data(), (float *)s->best_beta.p + J.chain_offset,
+                        J.n_chains * 4u, hipMemcpyDeviceToHost));
+  uint32_t best = 0u;
+  for (uint32_t c = 1; c < J.n_chains; c++)
+    if (st[c] != 0xffffffffu && sc[c] > sc[best]) best = c;
+  bbs_job_result r;
+  std::memset(&r, 0, sizeof r);
+  r.id = j->id;
+  r.n_genes = j->n_genes;
+  r.n_cells = J.n_cells;
+  r.n_chains = J.n_chains;
+  r.n_steps = j->n_steps;
+  r.best_score = sc[best];
+  r.best_beta = bb[best];
+  r.best_chain = best;
+  r.best_step = st[best];
+  r.submit_s = j->t_submit;
+  r.admit_s = j->t_admit;
+  r.done_s = t_done;
+  const size_t off = (size_t)(J.row_offset + (uint64_t)best * J.n_nodes) * 16u;
+  SHIP_TRY(s, hipMemcpy(r.best_am, (char *)s->best_am.p + off, (size_t)J.n_nodes * 16u,
+                        hipMemcpyDeviceToHost));
+  SHIP_TRY(s, hipMemcpy(r.best_dm, (char *)s->best_dm.p + off, (size_t)J.n_nodes * 16u,
+                        hipMemcpyDeviceToHost));
+  s->done.push_back(r);
+  srv_free_job(s, j);
+  return BBS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void bbs_server_config_default(bbs_server_config *cfg) {
+  if (!cfg) return;
+  cfg->hbm_budget_bytes = 1ull << 30;
+  cfg->round_cost = 4000000u;
+  cfg->max_resident_chains = 0u;
+  cfg->waves_per_group = 1u;
+  cfg->n_streams = 4u;
+}
+
+const char *bbs_server_last_error(const bbs_server *s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+void bbs_server_destroy(bbs_server *s) {
+  if (!s) return;
+  if (s->device >= 0) (void)hipSetDevice(s->device);
+  for (int i = 0; i < 4; i++)
+    if (s->stream[i]) (void)hipStreamSynchronize(s->stream[i]);
+  for (SrvJob *j : s->pending) delete j;
+  for (SrvJob *j : s->resident) delete j;
+  DevBuf *bufs[] = {&s->am,        &s->dm,         &s->best_am,   &s->best_dm,
+                    &s->beta,      &s->score,      &s->best_beta, &s->best_score,
+                    &s->best_step, &s->cells,      &s->d_jobs,    &s->d_items};
+  for (DevBuf *d : bufs) d->release();
+  for (int i = 0; i < 4; i++) {
+    if (s->done_ev[i]) (void)hipEventDestroy(s->done_ev[i]);
+    if (s->stream[i]) (void)hipStreamDestroy(s->stream[i]);
+  }
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  delete s;
+}
+
+int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
+  if (!out) return sfail(nullptr, BBS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  bbs_server_config c;
+  if (cfg)
+    c = *cfg;
+  else
+    bbs_server_config_default(&c);
+  if (c.waves_per_group < 1u || c.waves_per_group > 4u || c.n_streams < 1u ||
+      c.n_streams > 4u || c.round_cost == 0u || c.hbm_budget_bytes < (1ull << 20))
+    return sfail(nullptr, BBS_ERR_ARG, "invalid server configuration");
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
+    return sfail(nullptr, BBS_ERR_NODEV, "no HIP device available");
+  int dev = -1;
+  for (int d = 0; d < n_dev && dev < 0; d++)
+    if (is_gfx950(d)) dev = d;
+  if (dev < 0) return sfail(nullptr, BBS_ERR_NODEV, "no gfx950 device found");
+  bbs_server *s = new bbs_server();
+  s->cfg = c;
+  s->device = dev;
+  s->t0 = std::chrono::steady_clock::now();
+  hipDeviceProp_t prop;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+  for (uint32_t i = 0; i < c.n_streams && e == hipSuccess; i++) {
+    e = hipStreamCreateWithFlags(&s->stream[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done_ev[i], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+  if (e != hipSuccess) {
+    bbs_server_destroy(s);
+    return sfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
+  }
+  s->n_cus = (uint32_t)prop.multiProcessorCount;
+  s->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
+  /* as many chain waves as run at once: LDS per CU, and two waves per SIMD
+   * (a chain wave uses up to 256 VGPRs) */
+  s->lds_cap = (uint64_t)s->n_cus * s->lds_per_cu;
+  s->chain_cap = c.max_resident_chains ? c.max_resident_chains : s->n_cus * 8u;
+  /* device memory: 64 B per tree row (am, dm and their bests), 20 B per
+   * chain, 32 B per cell; rows take 90 % of the budget */
+  const uint64_t B = c.hbm_budget_bytes;
+  const uint64_t n_rows = B / 10u * 9u / 64u, n_chains = B / 50u / 20u, n_cells = B / 50u / 32u;
+  s->rows.init(n_rows);
+  s->chains.init(n_chains);
+  s->cellsA.init(n_cells);
+  int rc;
+  if ((rc = srv_alloc(s, s->am, n_rows * 16u)) || (rc = srv_alloc(s, s->dm, n_rows * 16u)) ||
+      (rc = srv_alloc(s, s->best_am, n_rows * 16u)) ||
+      (rc = srv_alloc(s, s->best_dm, n_rows * 16u)) ||
+      (rc = srv_alloc(s, s->beta, n_chains * 4u)) ||
+      (rc = srv_alloc(s, s->score, n_chains * 4u)) ||
+      (rc = srv_alloc(s, s->best_beta, n_chains * 4u)) ||
+      (rc = srv_alloc(s, s->best_score, n_chains * 4u)) ||
+      (rc = srv_alloc(s, s->best_step, n_chains * 4u)) ||
+      (rc = srv_alloc(s, s->cells, n_cells * 32u))) {
+    std::string why = s->err;
+    bbs_server_destroy(s);
+    return sfail(nullptr, rc, "%s", why.c_str());
+  }
+  *out = s;
+  return BBS_OK;
+}
+
+int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id) {
+  if (!s || !job) return sfail(s, BBS_ERR_ARG, "NULL argument");
+  SrvJob *j = new SrvJob();
+  std::string why;
+  int rc = fill_job(*job, j->J, why);
+  if (rc) {
+    delete j;
+    return sfail(s, rc, "job rejected: %s", why.c_str());
+  }
+  if (job->params.n_steps == 0u || job->params.n_chains == 0u) {
+    delete j;
+    return sfail(s, BBS_ERR_ARG, "job rejected: no chains or no steps");
+  }
+  const uint32_t G = job->n_genes;
+  const uint64_t gmask[2] = {G >= 64u ? ~0ull : ((1ull << G) - 1ull),
+                             G <= 64u ? 0ull : ((1ull << (G - 64u)) - 1ull)};
+  j->cells.resize(4u * (size_t)job->n_cells);
+  for (uint32_t c = 0; c < job->n_cells; c++)
+    for (int wd = 0; wd < 2; wd++) {
+      const uint64_t K = job->is_known[2u * c + wd] & gmask[wd];
+      const uint64_t M = job->is_mutated[2u * c + wd];
+      j->cells[4u * c + wd] = K & M;
+      j->cells[4u * c + 2u + wd] = K & ~M;
+    }
+  if ((uint64_t)j->J.n_chains * j->J.n_nodes > s->rows.cap || j->J.n_chains > s->chains.cap ||
+      j->J.n_cells > s->cellsA.cap) {
+    delete j;
+    return sfail(s, BBS_ERR_BUDGET, "job rejected: larger than the server's device memory budget");
+  }
+  j->n_genes = G;
+  j->n_steps = job->params.n_steps;
+  j->cost = step_cost(j->J);
+  const uint32_t f = bbs::batch_wave_floats(j->J.shape, j->J.n_nodes);
+  j->lds = f ? 4ull * f : 2560ull;
+  j->t_submit = srv_now(s);
+  j->id = s->next_id++;
+  s->pending.push_back(j);
+  if (id) *id = j->id;
+  return BBS_OK;
+}
+
+int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
+                    uint32_t *n_done_ready) {
+  if (!s) return sfail(nullptr, BBS_ERR_ARG, "server is NULL");
+  SHIP_TRY(s, hipSetDevice(s->device));
+  /* admission in arrival order */
+  while (!s->pending.empty()) {
+    bool ok = false;
+    int rc = srv_admit(s, ok);
+    if (rc) return rc;
+    if (!ok) break;
+  }
+  if (!s->resident.empty()) {
+    const double t_round = srv_now(s);
+    /* this round's windows: equal work per chain in step-cost units */
+    std::vector<bbs::BatchJob> jobs(s->resident.size());
+    std::vector<float> ninf;
+    for (size_t r = 0; r < s->resident.size(); r++) {
+      SrvJob *j = s->resident[r];
+      bbs::BatchJob J = j->J;
+      const uint64_t adv = std::max<uint64_t>(1u, (uint64_t)(s->cfg.round_cost / j->cost));
+      J.step_begin = j->steps_done;
+      J.step_end = (uint32_t)std::min<uint64_t>(j->n_steps, j->steps_done + adv);
+      J.score_init = j->started ? 0u : 1u;
+      jobs[r] = J;
+    }
+    SHIP_TRY(s, hipEventRecord(s->ev0, s->stream[0]));
+    /* newly admitted jobs: initial trees as bbs_begin draws them, no maxima
+     * (the -inf source stays allocated until the round has synchronised) */
+    uint32_t new_chains = 0;
+    for (SrvJob *j : s->resident)
+      if (!j->started) new_chains = std::max(new_chains, j->J.n_chains);
+    ninf.assign(new_chains, -INFINITY);
+    for (SrvJob *j : s->resident) {
+      if (j->started) continue;
+      const bbs::BatchJob &J = j->J;
+      bbs::TreeGenArgs g{};
+      g.am = (uint64_t *)s->am.p + 2ull * J.row_offset;
+      g.dm = (uint64_t *)s->dm.p + 2ull * J.row_offset;
+      g.beta = (float *)s->beta.p + J.chain_offset;
+      g.beta_value = J.beta_mean;
+      g.seed = J.seed;
+      g.job = J.job_tag;
+      g.chain_offset = 0u;
+      g.purpose = BBS_PURPOSE_INIT_TREE;
+      g.n_trees = J.n_chains;
+      g.n_nodes = J.n_nodes;
+      SHIP_TRY(s, bbs::launch_random_trees(g, s->stream[0]));
+      SHIP_TRY(s, hipMemsetAsync((uint32_t *)s->best_step.p + J.chain_offset, 0xff,
+                                 J.n_chains * 4u, s->stream[0]));
+      SHIP_TRY(s, hipMemcpyAsync((float *)s->best_score.p + J.chain_offset, ninf.data(),
+                                 J.n_chains * 4u, hipMemcpyHostToDevice, s->stream[0]));
+      SHIP_TRY(s, hipMemcpyAsync((float *)s->best_beta.p + J.chain_offset, ninf.data(),
+                                 J.n_chains * 4u, hipMemcpyHostToDevice, s->stream[0]));
+    }
+    /* items grouped by register layout, largest work first */
+    std::vector<uint32_t> order(jobs.size());
+    for (uint32_t r = 0; r < order.size(); r++) order[r] = r;
+    auto work = [&](uint32_t r) {
+      return s->resident[r]->cost * (double)(jobs[r].step_end - jobs[r].step_begin);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+      if (jobs[x].shape != jobs[y].shape) return jobs[x].shape < jobs[y].shape;
+      return work(x) > work(y);
+    });
+    std::vector<uint32_t> items;
+    struct R {
+      uint32_t shape, first, count, max_nodes;
+      double work;
+    };
+    std::vector<R> ranges;
+    for (size_t i = 0; i < order.size();) {
+      R rg{jobs[order[i]].shape, (uint32_t)items.size(), 0u, 0u, 0.0};
+      for (; i < order.size() && jobs[order[i]].shape == rg.shape; i++) {
+        const uint32_t r = order[i];
+        for (uint32_t c = 0; c < jobs[r].n_chains; c++) items.push_back((r << 16) | c);
+        rg.max_nodes = std::max(rg.max_nodes, jobs[r].n_nodes);
+        rg.work += work(r) * jobs[r].n_chains;
+      }
+      rg.count = (uint32_t)items.size() - rg.first;
+      ranges.push_back(rg);
+    }
+    std::stable_sort(ranges.begin(), ranges.end(),
+                     [](const R &x, const R &y) { return x.work > y.work; });
+    int rc;
+    if ((rc = srv_reserve(s, s->d_jobs, jobs.size() * sizeof(bbs::BatchJob))) ||
+        (rc = srv_reserve(s, s->d_items, items.size() * 4u)))
+      return rc;
+    SHIP_TRY(s, hipMemcpyAsync(s->d_jobs.p, jobs.data(), jobs.size() * sizeof(bbs::BatchJob),
+                               hipMemcpyHostToDevice, s->stream[0]));
+    SHIP_TRY(s, hipMemcpyAsync(s->d_items.p, items.data(), items.size() * 4u,
+                               hipMemcpyHostToDevice, s->stream[0]));
+    bbs::BatchArgs a{};
+    a.jobs = (const bbs::BatchJob *)s->d_jobs.p;
+    a.items = (const uint32_t *)s->d_items.p;
+    a.n_items = (uint32_t)items.size();
+    a.cells = (const uint64_t *)s->cells.p;
+    a.am = (uint64_t *)s->am.p;
+    a.dm = (uint64_t *)s->dm.p;
+    a.best_am = (uint64_t *)s->best_am.p;
+    a.best_dm = (uint64_t *)s->best_dm.p;
+    a.beta = (float *)s->beta.p;
+    a.score = (float *)s->score.p;
+    a.best_beta = (float *)s->best_beta.p;
+    a.best_score = (float *)s->best_score.p;
+    a.best_step = (uint32_t *)s->best_step.p;
+    /* layouts spread over the streams after the uploads on stream 0 */
+    const uint32_t ns = s->cfg.n_streams;
+    SHIP_TRY(s, hipEventRecord(s->done_ev[0], s->stream[0]));
+    for (uint32_t i = 1; i < ns; i++) SHIP_TRY(s, hipStreamWaitEvent(s->stream[i], s->done_ev[0], 0));
+    const uint32_t wpg = s->cfg.waves_per_group;
+    for (size_t r = 0; r < ranges.size(); r++) {
+      const R &rg = ranges[r];
+      uint32_t f = bbs::batch_wave_floats(rg.shape, rg.max_nodes);
+      if (4ull * f * wpg > 65536ull) f = 0u;
+      SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg, f,
+                                          s->stream[r % ns]));
+    }
+    for (uint32_t i = 1; i < ns; i++) {
+      SHIP_TRY(s, hipEventRecord(s->done_ev[i], s->stream[i]));
+      SHIP_TRY(s, hipStreamWaitEvent(s->stream[0], s->done_ev[i], 0));
+    }
+    SHIP_TRY(s, hipEventRecord(s->ev1, s->stream[0]));
+    SHIP_TRY(s, hipEventSynchronize(s->ev1));
+    const double t_end = srv_now(s);
+    s->busy_s += t_end - t_round;
+    s->rounds++;
+    /* advance and retire */
+    std::vector<SrvJob *> still;
+    for (size_t r = 0; r < s->resident.size(); r++) {
+      SrvJob *j = s->resident[r];
+      s->steps_total += (uint64_t)jobs[r].n_chains * (jobs[r].step_end - jobs[r].step_begin);
+      j->steps_done = jobs[r].step_end;
+      j->started = true;
+      if (j->steps_done >= j->n_steps) {
+        rc = srv_retire(s, j, t_end);
+        if (rc) return rc;
+      } else {
+        still.push_back(j);
+      }
+    }
+    s->resident.swap(still);
+  }
+  if (n_resident) *n_resident = (uint32_t)s->resident.size();
+  if (n_pending) *n_pending = (uint32_t)s->pending.size();
+  if (n_done_ready) *n_done_ready = (uint32_t)s->done.size();
+  return BBS_OK;
+}
+
+int bbs_server_poll(bbs_server *s, bbs_job_result *out, uint32_t max, uint32_t *n_out) {
+  if (!s || !n_out || (max && !out)) return sfail(s, BBS_ERR_ARG, "NULL argument");
+  uint32_t n = 0;
+  while (n < max && !s->done.empty()) {
+    out[n++] = s->done.front();
+    s->done.pop_front();
+  }
+  *n_out = n;
+  return BBS_OK;
+}
+
+double bbs_server_now(const bbs_server *s) { return s ? srv_now(s) : 0.0; }
+
+int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
+                     uint64_t *rounds) {
+  if (!s) return sfail(nullptr, BBS_ERR_ARG, "server is NULL");
+  if (chain_steps) *chain_steps = s->steps_total;
+  if (busy_s) *busy_s = s->busy_s;
+  if (rounds) *rounds = s->rounds;
+  return BBS_OK;
 }
 
 } /* extern "C" */

@@ -204,6 +204,73 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
     }
 
 
+def config5_jobs(n_jobs, seed=5):
+    """The served stream: Poisson arrivals at 50 jobs/s; genes U[16,127],
+    cells U[16,128], chains U[8,256], steps log-uniform in [1e4, 1e6]
+    (numpy generator, seed 5); data as the 63 x 64 configuration (-ad 0.2,
+    -fd 1e-4, 10 % missing) from seed 1000 + i."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / 50.0)
+        G = int(rng.integers(16, 128))
+        Cc = int(rng.integers(16, 129))
+        R = int(rng.integers(8, 257))
+        S = int(10 ** rng.uniform(4.0, 6.0))
+        m, _, _ = B.generate_dataset(1000 + i, G, Cc, 1e-4, 0.2, 0.1)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=R, n_steps=S, seed=1000 + i, alpha_mean=1e-4,
+                             beta_mean=0.2)
+        jobs.append((t, (p, mut, kno, G)))
+    return jobs
+
+
+def serving_bench(n_jobs):
+    """Continuous batching on the first n_jobs of the served stream, in real
+    time: each job is submitted at its arrival time; latency = completion -
+    arrival."""
+    jobs = config5_jobs(n_jobs)
+    lat = []
+    arrival = {}
+    with B.Server(B.server_config(hbm_budget_bytes=4 << 30)) as srv:
+        t0 = srv.now()
+        nxt = 0
+        done = 0
+        while done < n_jobs:
+            now = srv.now() - t0
+            while nxt < n_jobs and jobs[nxt][0] <= now:
+                jid = srv.submit(*jobs[nxt][1])
+                arrival[jid] = jobs[nxt][0]
+                nxt += 1
+            res, pend, ready = srv.step()
+            for r in srv.poll():
+                lat.append(r.done_s - t0 - arrival[r.id])
+                done += 1
+            if res == 0 and pend == 0 and nxt < n_jobs:
+                time.sleep(max(0.0, jobs[nxt][0] - (srv.now() - t0)))
+        end = srv.now() - t0
+        chain_steps, busy, rounds = srv.stats()
+    span = jobs[-1][0]
+    offered = sum(p.n_chains * p.n_steps for _, (p, *_r) in jobs)
+    lat = np.array(lat)
+    return {
+        "workload": f"first {n_jobs} jobs of the served stream (Poisson 50 jobs/s, seed 5; "
+                    "genes U[16,127], cells U[16,128], chains U[8,256], steps log-uniform "
+                    "[1e4,1e6])",
+        "jobs": n_jobs,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "max_ms": float(lat.max() * 1e3),
+        "aggregate_steps_per_s": chain_steps / end,
+        "busy_steps_per_s": chain_steps / busy,
+        "offered_steps_per_s": offered / span,
+        "arrival_span_s": span,
+        "makespan_s": end,
+        "rounds": rounds,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +282,8 @@ def main():
     ap.add_argument("--no-score", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=200_000)
+    ap.add_argument("--serve-jobs", type=int, default=200,
+                    help="jobs of the served stream to run (0: skip serving)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -272,6 +341,11 @@ def main():
     if score:
         log(f"[bench] scoring: {score['trees_per_s']:.3e} trees/s")
     batched = None if a.no_batch else batched_bench(a.batch_steps)
+    serving = serving_bench(a.serve_jobs) if a.serve_jobs > 0 else None
+    if serving:
+        log(f"[bench] served {serving['jobs']} jobs: p50 {serving['p50_ms']:.0f} ms, "
+            f"p99 {serving['p99_ms']:.0f} ms, {serving['aggregate_steps_per_s']:.3e} steps/s "
+            f"(offered {serving['offered_steps_per_s']:.3e})")
     if batched:
         log(f"[bench] batched {batched['batched_steps_per_s']:.3e} steps/s, plain "
             f"{batched['plain_steps_per_s']:.3e} steps/s, x{batched['speedup']:.2f}")
@@ -318,7 +392,8 @@ def main():
         "best_beta": best_beta,
         "scoring": score,
         "batched": batched,
-        "serving_p99_ms": None,
+        "serving_p99_ms": serving["p99_ms"] if serving else None,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -251,6 +251,55 @@ void bbs_batch_destroy(bbs_batch *b);
 int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
                        bbs_batch_config *best, float *best_ms);
 
+/* ---- serving: continuous batching of tree-search jobs ------------------
+ * No reference counterpart: the reference runs one dataset per invocation
+ * (src/main.cpp:159-169). A server keeps every admitted job's chains and
+ * best trees on the device and advances all of them together, round by
+ * round; jobs that arrive meanwhile are admitted as device memory and
+ * concurrent-wave capacity free up. Every random draw is keyed by
+ * (seed, job_tag, chain, step), so a job's result is the same alone
+ * (bbs_create + bbs_run with job_tag 0) and in any mix. */
+typedef struct bbs_server_config {
+  uint64_t hbm_budget_bytes;    /* device memory for resident jobs' state  */
+  uint32_t round_cost;          /* work per round and chain in step-cost
+                                   units (about one per VALU instruction);
+                                   a job advances round_cost / its step
+                                   cost steps per round (at least 1)       */
+  uint32_t max_resident_chains; /* 0: as many chain waves as the device
+                                   runs at once (LDS and registers)        */
+  uint32_t waves_per_group;     /* chain waves per workgroup, 1..4          */
+  uint32_t n_streams;           /* 1..4                                     */
+} bbs_server_config;
+
+typedef struct bbs_job_result {
+  uint64_t id;
+  uint32_t n_genes, n_cells, n_chains, n_steps;
+  float best_score, best_beta;
+  uint32_t best_chain, best_step; /* chain-major first maximum          */
+  double submit_s, admit_s, done_s; /* seconds on the server clock      */
+  uint64_t best_am[BBS_MAX_NODES][2], best_dm[BBS_MAX_NODES][2];
+} bbs_job_result;
+
+typedef struct bbs_server bbs_server;
+void bbs_server_config_default(bbs_server_config *cfg);
+int bbs_server_create(const bbs_server_config *cfg, bbs_server **out);
+/* Copies the job (parameters and data); *id identifies its result. */
+int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id);
+/* One round: admit pending jobs in arrival order while they fit, advance
+ * every resident job, retire finished ones. Blocks until the round's
+ * kernels are done. Counts may be NULL. */
+int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
+                    uint32_t *n_done_ready);
+/* Moves up to max finished results (in completion order) to out. */
+int bbs_server_poll(bbs_server *s, bbs_job_result *out, uint32_t max, uint32_t *n_out);
+/* Seconds since the server was created (the clock of the results). */
+double bbs_server_now(const bbs_server *s);
+/* Chain steps run so far, and the summed duration of the rounds. */
+int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
+                     uint64_t *rounds);
+const char *bbs_server_last_error(const bbs_server *s);
+void bbs_server_destroy(bbs_server *s);
+
 /* Message of the last failure on ctx, or of the last failed bbs_create on
  * this thread when ctx is NULL. */
 const char *bbs_last_error(const bbs_ctx *ctx);

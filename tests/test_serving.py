@@ -1,0 +1,99 @@
+"""The serving loop (continuous batching): jobs submitted at different
+times, admitted as capacity frees up and advanced round by round together
+with whatever else is resident, must give exactly the result of the same
+job run alone through the plain path (bbs_create + bbs_run)."""
+import numpy as np
+import pytest
+
+import bbscite_amd as B
+from bbscite_amd import _capi
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def test_server_rejects_bad_configuration():
+    """Checked before any device is touched."""
+    for kw in ({"waves_per_group": 5}, {"n_streams": 0}, {"round_cost": 0},
+               {"hbm_budget_bytes": 1024}):
+        with pytest.raises(B.BbsError) as e:
+            B.Server(B.server_config(**kw))
+        assert e.value.code == _capi.ERR_ARG
+
+
+def _job(i, G, C, chains, steps, p_beta=0.1):
+    m, _, _ = B.generate_dataset(300 + i, G, C, 1e-4, 0.2, 0.1)
+    mut, kno = B.encode_matrix(m)
+    p = B.default_params(n_chains=chains, n_steps=steps, seed=500 + i, alpha_mean=1e-4,
+                         beta_mean=0.2 + 0.02 * (i % 3), prob_beta_change=p_beta)
+    return p, mut, kno, G
+
+
+JOBS = [  # G, C, chains, steps: every register layout family, uneven lengths
+    (16, 16, 8, 900), (94, 95, 12, 700), (40, 80, 9, 1300), (127, 128, 5, 400),
+    (63, 64, 16, 1100), (20, 120, 7, 800), (100, 33, 10, 600), (2, 5, 3, 500),
+    (64, 65, 6, 1000), (30, 96, 11, 750),
+]
+
+
+@pytest.mark.gpu
+def test_served_results_equal_alone():
+    """Small rounds (many windows per job) and a small chain budget (jobs
+    queue and are admitted as others finish); some jobs arrive while others
+    run. Each result equals the job's plain run bit for bit."""
+    say("[serve] mixed jobs")
+    jobs = [_job(i, *spec) for i, spec in enumerate(JOBS)]
+    cfg = B.server_config(round_cost=300_000, max_resident_chains=40, n_streams=3,
+                          waves_per_group=2, hbm_budget_bytes=64 << 20)
+    results = {}
+    ids = {}
+    with B.Server(cfg) as srv:
+        for i in range(4):
+            ids[srv.submit(*jobs[i])] = i
+        rounds = 0
+        nxt = 4
+        while len(results) < len(jobs):
+            if nxt < len(jobs) and rounds % 3 == 2:
+                ids[srv.submit(*jobs[nxt])] = nxt
+                nxt += 1
+            res, pend, ready = srv.step()
+            rounds += 1
+            for r in srv.poll():
+                results[ids[r.id]] = r
+            assert rounds < 20000
+        chain_steps, busy, n_rounds = srv.stats()
+    assert chain_steps == sum(p.n_chains * p.n_steps for p, *_ in jobs)
+    assert n_rounds > len(jobs)
+    say("[serve] plain runs")
+    for i, (p, mut, kno, G) in enumerate(jobs):
+        r = results[i]
+        assert r.submit_s <= r.admit_s <= r.done_s
+        with B.Context(p, mut, kno, G) as ctx:
+            ctx.run()
+            am, dm, beta, score = ctx.best()
+            bs, st, _, _ = ctx.chain_results()
+        assert _bits(r.best_score) == _bits(score), i
+        assert _bits(r.best_beta) == _bits(beta), i
+        assert np.array_equal(r.rows("am"), am) and np.array_equal(r.rows("dm"), dm), i
+        assert r.best_step == st[r.best_chain]
+
+
+@pytest.mark.gpu
+def test_server_rejects_jobs_beyond_its_memory():
+    say("[serve] budget")
+    with B.Server(B.server_config(hbm_budget_bytes=1 << 20)) as srv:
+        with pytest.raises(B.BbsError) as e:
+            srv.submit(*_job(0, 127, 128, 200, 10))
+        assert e.value.code == _capi.ERR_BUDGET
+        jid = srv.submit(*_job(1, 10, 12, 4, 50))
+        while True:
+            srv.step()
+            out = srv.poll()
+            if out:
+                assert out[0].id == jid
+                break
