@@ -15,6 +15,9 @@ from ._capi import (  # noqa: F401
     generate_dataset,
     initial_tree,
     lib,
+    newick_to_parent_vector,
+    score_attachments,
+    tost,
     parent_vector_to_matrix,
     parse_csv,
     pruefer_to_parent_vector,

@@ -184,6 +184,11 @@ _SIGS = {
         C.c_int,
         [C.POINTER(Job), _u32, _u32, C.POINTER(BatchConfig), C.POINTER(C.c_float)],
     ),
+    "bbs_newick_to_parent_vector": (C.c_int, [C.c_char_p, _P, C.POINTER(_u32)]),
+    "bbs_score_attachments": (C.c_int, [_P, _u32, _P, _P, _u32, C.c_double, C.c_double, _P,
+                                        C.POINTER(C.c_double)]),
+    "bbs_tost": (C.c_int, [_P, _u32, C.c_double, C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bbs_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "bbs_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_P)]),
     "bbs_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
@@ -339,6 +344,37 @@ def tree_to_graphviz(am, dm):
 
 def tree_to_newick(am, dm):
     return _text(lib().bbs_tree_to_newick, am, dm)
+
+
+def newick_to_parent_vector(text):
+    """Newick text -> parent vector (root last, parent[root] = root)."""
+    if isinstance(text, str):
+        text = text.encode()
+    par = np.zeros(MAX_NODES, np.uint32)
+    n = _u32()
+    _check(lib().bbs_newick_to_parent_vector(text, _ptr(par), C.byref(n)))
+    return par[: n.value].copy()
+
+
+def score_attachments(dm, is_mutated, is_known, alpha, beta):
+    """(log-likelihood in double, attachment node per cell) of a tree with
+    every cell at its most likely node."""
+    dm = np.ascontiguousarray(dm, np.uint64)
+    mut = np.ascontiguousarray(is_mutated, np.uint64)
+    kno = np.ascontiguousarray(is_known, np.uint64)
+    att = np.zeros(mut.shape[0], np.uint32)
+    ll = C.c_double()
+    _check(lib().bbs_score_attachments(_ptr(dm), dm.shape[0], _ptr(mut), _ptr(kno), mut.shape[0],
+                                       alpha, beta, _ptr(att), C.byref(ll)))
+    return ll.value, att
+
+
+def tost(diff, delta):
+    """Two one-sided t-tests against +-delta: (mean, p_lower, p_upper)."""
+    d = np.ascontiguousarray(diff, np.float64)
+    m, pl, pu = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().bbs_tost(_ptr(d), len(d), delta, C.byref(m), C.byref(pl), C.byref(pu)))
+    return m.value, pl.value, pu.value
 
 
 # ---------------------------------------------------------------- device

@@ -1,4 +1,5 @@
-"""In-tree build of the HIP library (libbbscite.so) and the CLI (bbscite).
+"""In-tree build of the HIP library (libbbscite.so), the CLI (bbscite) and
+the quality harness (bbscite_quality).
 
 Everything is compiled for gfx950 with hipcc; no JIT cache is used, so the
 built files travel with the source tree.
@@ -13,6 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libbbscite.so")
 CLI = os.path.join(HERE, "bbscite")
+QUALITY = os.path.join(HERE, "bbscite_quality")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -48,16 +50,21 @@ def build_lib():
     return LIB
 
 
-def build_cli():
-    src = os.path.join(HERE, "cli", "bbscite_main.cpp")
-    tmp = CLI + ".tmp"
+def _build_tool(src_name, out):
+    src = os.path.join(HERE, "cli", src_name)
+    tmp = out + ".tmp"
     _run([
         "g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, src,
         f"-I{os.path.join(ROOT, 'include')}", f"-L{HERE}", "-lbbscite",
         f"-Wl,-rpath,$ORIGIN",
     ])
-    os.replace(tmp, CLI)
-    return CLI
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli():
+    _build_tool("bbscite_quality.cpp", QUALITY)
+    return _build_tool("bbscite_main.cpp", CLI)
 
 
 def build_all():

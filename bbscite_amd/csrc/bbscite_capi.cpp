@@ -2073,3 +2073,191 @@ int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
 }
 
 } /* extern "C" */
+
+/* ------------------------------------------------------------ quality */
+
+namespace {
+
+struct NewickParser {
+  const char *p;
+  std::vector<std::pair<uint64_t, uint64_t>> edges; /* (parent, child) labels */
+  std::vector<uint64_t> labels;
+  bool ok = true;
+  void ws() {
+    while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') p++;
+  }
+  bool number(uint64_t &v) {
+    ws();
+    if (*p < '0' || *p > '9') return false;
+    v = 0;
+    while (*p >= '0' && *p <= '9') {
+      v = v * 10u + (uint64_t)(*p - '0');
+      if (v > 1000000000ull) return false;
+      p++;
+    }
+    return true;
+  }
+  /* tree : NUMBER | "(" tree ("," tree)* ")" NUMBER */
+  bool tree(uint64_t &root, int depth) {
+    if (depth > 256) return false;
+    ws();
+    if (*p == '(') {
+      p++;
+      std::vector<uint64_t> kids;
+      for (;;) {
+        uint64_t k;
+        if (!tree(k, depth + 1)) return false;
+        kids.push_back(k);
+        ws();
+        if (*p == ',') {
+          p++;
+          continue;
+        }
+        if (*p == ')') {
+          p++;
+          break;
+        }
+        return false;
+      }
+      if (!number(root)) return false;
+      for (uint64_t k : kids) edges.emplace_back(root, k);
+    } else if (!number(root)) {
+      return false;
+    }
+    labels.push_back(root);
+    return true;
+  }
+};
+
+/* regularized incomplete beta I_x(a, b) by its continued fraction
+ * (Abramowitz & Stegun 26.5.8), evaluated with the modified Lentz method */
+double inc_beta(double a, double b, double x) {
+  if (x <= 0.0) return 0.0;
+  if (x >= 1.0) return 1.0;
+  if (x > (a + 1.0) / (a + b + 2.0)) return 1.0 - inc_beta(b, a, 1.0 - x);
+  const double tiny = 1e-300;
+  const double lbeta = std::lgamma(a) + std::lgamma(b) - std::lgamma(a + b);
+  const double front = std::exp(a * std::log(x) + b * std::log1p(-x) - lbeta) / a;
+  double f = 1.0, c = 1.0, d = 0.0;
+  for (int i = 0; i <= 400; i++) {
+    const int m = i / 2;
+    double num;
+    if (i == 0)
+      num = 1.0;
+    else if (i % 2 == 0)
+      num = (m * (b - m) * x) / ((a + 2.0 * m - 1.0) * (a + 2.0 * m));
+    else
+      num = -((a + m) * (a + b + m) * x) / ((a + 2.0 * m) * (a + 2.0 * m + 1.0));
+    d = 1.0 + num * d;
+    if (std::fabs(d) < tiny) d = tiny;
+    d = 1.0 / d;
+    c = 1.0 + num / c;
+    if (std::fabs(c) < tiny) c = tiny;
+    const double cd = c * d;
+    f *= cd;
+    if (std::fabs(1.0 - cd) < 1e-15) break;
+  }
+  return front * (f - 1.0);
+}
+
+/* Student t distribution with nu degrees of freedom: P(T <= t), with the
+ * tail beyond |t| computed directly (no 1 - p cancellation for small p) */
+double student_cdf(double t, double nu) {
+  const double x = nu / (nu + t * t);
+  const double tail = 0.5 * inc_beta(0.5 * nu, 0.5, x);
+  return t > 0.0 ? 1.0 - tail : tail;
+}
+double student_sf(double t, double nu) { return student_cdf(-t, nu); }
+
+} // namespace
+
+extern "C" {
+
+int bbs_newick_to_parent_vector(const char *text, uint32_t *parent, uint32_t *n_nodes) {
+  if (!text || !parent || !n_nodes) return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  NewickParser P{text, {}, {}};
+  uint64_t root = 0;
+  if (!P.tree(root, 0)) return fail(nullptr, BBS_ERR_ARG, "malformed Newick text");
+  P.ws();
+  if (*P.p != '\0') return fail(nullptr, BBS_ERR_ARG, "trailing characters after the tree");
+  const size_t n = P.labels.size();
+  if (n < 1u || n > BBS_MAX_NODES) return fail(nullptr, BBS_ERR_SIZE, "%zu nodes", n);
+  std::vector<uint64_t> sorted = P.labels;
+  std::sort(sorted.begin(), sorted.end());
+  for (size_t i = 1; i < n; i++)
+    if (sorted[i] == sorted[i - 1])
+      return fail(nullptr, BBS_ERR_ARG, "label %llu appears twice",
+                  (unsigned long long)sorted[i]);
+  auto idx = [&](uint64_t label) {
+    return (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), label) - sorted.begin());
+  };
+  if (idx(root) != n - 1u)
+    return fail(nullptr, BBS_ERR_ARG, "the root must carry the largest label");
+  for (size_t i = 0; i < n; i++) parent[i] = (uint32_t)(n - 1u);
+  for (const auto &e : P.edges) parent[idx(e.second)] = idx(e.first);
+  *n_nodes = (uint32_t)n;
+  return BBS_OK;
+}
+
+int bbs_score_attachments(const uint64_t *dm, uint32_t n_nodes, const uint64_t *is_mutated,
+                          const uint64_t *is_known, uint32_t n_cells, double alpha,
+                          double beta, uint32_t *attachment, double *log_likelihood) {
+  if (!dm || !is_mutated || !is_known || !log_likelihood)
+    return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  if (n_nodes < 2u || n_nodes > BBS_MAX_NODES || n_cells < 1u || n_cells > BBS_MAX_CELLS)
+    return fail(nullptr, BBS_ERR_SIZE, "unsupported size");
+  const uint32_t G = n_nodes - 1u;
+  const uint64_t gmask[2] = {G >= 64u ? ~0ull : ((1ull << G) - 1ull),
+                             G <= 64u ? 0ull : ((1ull << (G - 64u)) - 1ull)};
+  const double l00 = std::log(1.0 - alpha), l10 = std::log(alpha), l01 = std::log(beta),
+               l11 = std::log(1.0 - beta);
+  double total = 0.0;
+  for (uint32_t c = 0; c < n_cells; c++) {
+    double best = 0.0;
+    uint32_t arg = 0u;
+    for (uint32_t v = 0; v < n_nodes; v++) {
+      uint32_t n[2][2] = {{0u, 0u}, {0u, 0u}}; /* [observed][true] */
+      for (int w = 0; w < 2; w++) {
+        const uint64_t K = is_known[2u * c + w] & gmask[w], M = is_mutated[2u * c + w];
+        const uint64_t A = dm[2u * v + w] & gmask[w];
+        n[1][1] += (uint32_t)__builtin_popcountll(K & M & A);
+        n[1][0] += (uint32_t)__builtin_popcountll(K & M & ~A);
+        n[0][1] += (uint32_t)__builtin_popcountll(K & ~M & A);
+        n[0][0] += (uint32_t)__builtin_popcountll(K & ~M & ~A);
+      }
+      const double ll = n[0][0] * l00 + n[1][0] * l10 + n[0][1] * l01 + n[1][1] * l11;
+      if (v == 0u || ll > best) {
+        best = ll;
+        arg = v;
+      }
+    }
+    if (attachment) attachment[c] = arg;
+    total += best;
+  }
+  *log_likelihood = total;
+  return BBS_OK;
+}
+
+int bbs_tost(const double *diff, uint32_t n, double delta, double *mean, double *p_lower,
+             double *p_upper) {
+  if (!diff || !p_lower || !p_upper) return fail(nullptr, BBS_ERR_ARG, "NULL argument");
+  if (n < 2u || !(delta > 0.0)) return fail(nullptr, BBS_ERR_ARG, "need n >= 2 and delta > 0");
+  double m = 0.0;
+  for (uint32_t i = 0; i < n; i++) m += diff[i];
+  m /= n;
+  double ss = 0.0;
+  for (uint32_t i = 0; i < n; i++) ss += (diff[i] - m) * (diff[i] - m);
+  const double sd = std::sqrt(ss / (n - 1u));
+  if (mean) *mean = m;
+  if (sd == 0.0) {
+    *p_lower = m > -delta ? 0.0 : 1.0;
+    *p_upper = m < delta ? 0.0 : 1.0;
+    return BBS_OK;
+  }
+  const double se = sd / std::sqrt((double)n), nu = (double)n - 1.0;
+  *p_lower = student_sf((m + delta) / se, nu);        /* H1: mean > -delta */
+  *p_upper = student_cdf((m - delta) / se, nu);       /* H1: mean < delta  */
+  return BBS_OK;
+}
+
+} /* extern "C" */

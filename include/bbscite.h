@@ -353,6 +353,27 @@ int64_t bbs_tree_to_graphviz(const uint64_t *am, const uint64_t *dm,
 int64_t bbs_tree_to_newick(const uint64_t *am, const uint64_t *dm,
                            uint32_t n_nodes, char *buf, uint64_t cap);
 
+/* ---- quality harness (verification/__main__.py:54-163 restated) ---------
+ * Newick text as written by bbs_tree_to_newick (or any labelling that
+ * src/MutationTree.hpp:880-894 produces) -> parent vector; labels that are
+ * not 0..n-1 are renumbered in sorted order (verification/lib.py:148-164);
+ * the root must carry the largest label. parent[root] = root. */
+int bbs_newick_to_parent_vector(const char *text, uint32_t *parent, uint32_t *n_nodes);
+/* Log-likelihood of a tree in double precision with each cell attached to
+ * its most likely node (verification/__main__.py:54-87,
+ * verification/lib.py:170-196 with the entry meanings of src/main.cpp:
+ * 0 = not mutated, 1 and 2 = mutated, 3 = missing; the first maximal node
+ * in index order wins). attachment may be NULL. */
+int bbs_score_attachments(const uint64_t *dm, uint32_t n_nodes, const uint64_t *is_mutated,
+                          const uint64_t *is_known, uint32_t n_cells, double alpha,
+                          double beta, uint32_t *attachment, double *log_likelihood);
+/* Two one-sided t-tests (verification/__main__.py:90-120) of n paired
+ * differences against the margin +-delta: p_lower for H0 mean <= -delta,
+ * p_upper for H0 mean >= delta; equivalence at level a when both are
+ * below a. All-equal differences give p = 0 (or 1) exactly. */
+int bbs_tost(const double *diff, uint32_t n, double delta, double *mean, double *p_lower,
+             double *p_upper);
+
 #ifdef __cplusplus
 }
 #endif

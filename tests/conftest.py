@@ -18,13 +18,15 @@ def pytest_configure(config):
 
     faulthandler.dump_traceback_later(150, repeat=True)
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
-    # Build only what is missing: the oracle (gcc) and the HIP library (hipcc).
+    # Build only what is missing: the oracle (gcc), the HIP library (hipcc)
+    # and the command-line tools (g++).
     from tests import _oracle  # noqa: F401
 
     if not os.path.exists(_oracle.LIB_PATH):
         _oracle.build()
-    lib = os.path.join(ROOT, "bbscite_amd", "libbbscite.so")
-    if not os.path.exists(lib):
-        from bbscite_amd import build
+    from bbscite_amd import build
 
+    if not os.path.exists(build.LIB):
         build.build_lib()
+    if not (os.path.exists(build.CLI) and os.path.exists(build.QUALITY)):
+        build.build_cli()
