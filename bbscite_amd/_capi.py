@@ -116,6 +116,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_uint64),
         ("round_cost", C.c_uint32),
+        ("round_ms", C.c_float),
         ("max_resident_chains", C.c_uint32),
         ("waves_per_group", C.c_uint32),
         ("n_streams", C.c_uint32),

@@ -1608,9 +1608,11 @@ struct SrvJob {
 struct bbs_server {
   int device = -1;
   bbs_server_config cfg{};
-  hipStream_t stream[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t done_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t stream[BBS_MAX_STREAMS] = {};
+  hipEvent_t done_ev[BBS_MAX_STREAMS] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> range_ev; /* start/end per layout range */
+  std::map<uint32_t, double> ms_per_cost; /* layout -> ms per (step x cost unit) */
   uint32_t n_cus = 0, lds_per_cu = 0;
   DevBuf am, dm, best_am, best_dm, beta, score, best_beta, best_score, best_step, cells,
       d_jobs, d_items;
@@ -1766,6 +1768,7 @@ void bbs_server_config_default(bbs_server_config *cfg) {
   if (!cfg) return;
   cfg->hbm_budget_bytes = 1ull << 30;
   cfg->round_cost = 4000000u;
+  cfg->round_ms = 20.0f;
   cfg->max_resident_chains = 0u;
   cfg->waves_per_group = 1u;
   cfg->n_streams = 4u;
@@ -1778,7 +1781,7 @@ const char *bbs_server_last_error(const bbs_server *s) {
 void bbs_server_destroy(bbs_server *s) {
   if (!s) return;
   if (s->device >= 0) (void)hipSetDevice(s->device);
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < BBS_MAX_STREAMS; i++)
     if (s->stream[i]) (void)hipStreamSynchronize(s->stream[i]);
   for (SrvJob *j : s->pending) delete j;
   for (SrvJob *j : s->resident) delete j;
@@ -1786,12 +1789,13 @@ void bbs_server_destroy(bbs_server *s) {
                     &s->beta,      &s->score,      &s->best_beta, &s->best_score,
                     &s->best_step, &s->cells,      &s->d_jobs,    &s->d_items};
   for (DevBuf *d : bufs) d->release();
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < BBS_MAX_STREAMS; i++) {
     if (s->done_ev[i]) (void)hipEventDestroy(s->done_ev[i]);
     if (s->stream[i]) (void)hipStreamDestroy(s->stream[i]);
   }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (hipEvent_t e : s->range_ev) (void)hipEventDestroy(e);
   delete s;
 }
 
@@ -1804,7 +1808,8 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
   else
     bbs_server_config_default(&c);
   if (c.waves_per_group < 1u || c.waves_per_group > 4u || c.n_streams < 1u ||
-      c.n_streams > 4u || c.round_cost == 0u || c.hbm_budget_bytes < (1ull << 20))
+      c.n_streams > (uint32_t)BBS_MAX_STREAMS || c.round_cost == 0u || !(c.round_ms > 0.0f) ||
+      c.hbm_budget_bytes < (1ull << 20))
     return sfail(nullptr, BBS_ERR_ARG, "invalid server configuration");
   int n_dev = 0;
   if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
@@ -1921,7 +1926,11 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
     for (size_t r = 0; r < s->resident.size(); r++) {
       SrvJob *j = s->resident[r];
       bbs::BatchJob J = j->J;
-      const uint64_t adv = std::max<uint64_t>(1u, (uint64_t)(s->cfg.round_cost / j->cost));
+      auto f = s->ms_per_cost.find(j->J.shape);
+      const double want = f == s->ms_per_cost.end()
+                              ? (double)s->cfg.round_cost / j->cost
+                              : (double)s->cfg.round_ms / (f->second * j->cost);
+      const uint64_t adv = (uint64_t)std::max(1.0, std::min(want, 4.0e9));
       J.step_begin = j->steps_done;
       J.step_end = (uint32_t)std::min<uint64_t>(j->n_steps, j->steps_done + adv);
       J.score_init = j->started ? 0u : 1u;
@@ -2012,12 +2021,19 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
     SHIP_TRY(s, hipEventRecord(s->done_ev[0], s->stream[0]));
     for (uint32_t i = 1; i < ns; i++) SHIP_TRY(s, hipStreamWaitEvent(s->stream[i], s->done_ev[0], 0));
     const uint32_t wpg = s->cfg.waves_per_group;
+    while (s->range_ev.size() < 2u * ranges.size()) {
+      hipEvent_t e = nullptr;
+      SHIP_TRY(s, hipEventCreate(&e));
+      s->range_ev.push_back(e);
+    }
     for (size_t r = 0; r < ranges.size(); r++) {
       const R &rg = ranges[r];
       uint32_t f = bbs::batch_wave_floats(rg.shape, rg.max_nodes);
       if (4ull * f * wpg > 65536ull) f = 0u;
+      SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r], s->stream[r % ns]));
       SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg, f,
                                           s->stream[r % ns]));
+      SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r + 1u], s->stream[r % ns]));
     }
     for (uint32_t i = 1; i < ns; i++) {
       SHIP_TRY(s, hipEventRecord(s->done_ev[i], s->stream[i]));
@@ -2028,6 +2044,26 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
     const double t_end = srv_now(s);
     s->busy_s += t_end - t_round;
     s->rounds++;
+    /* the measured time of each layout's launch, per step and cost unit of
+     * its heaviest job, sizes that layout's next slices */
+    for (size_t r = 0; r < ranges.size(); r++) {
+      float ms = 0.f;
+      SHIP_TRY(s, hipEventElapsedTime(&ms, s->range_ev[2u * r], s->range_ev[2u * r + 1u]));
+      double heaviest = 0.0;
+      for (uint32_t i = ranges[r].first; i < ranges[r].first + ranges[r].count;) {
+        const uint32_t jr = items[i] >> 16;
+        heaviest = std::max(heaviest, work(jr));
+        i += jobs[jr].n_chains;
+      }
+      if (heaviest > 0.0 && ms > 0.f) {
+        const double sample = ms / heaviest;
+        auto f = s->ms_per_cost.find(ranges[r].shape);
+        if (f == s->ms_per_cost.end())
+          s->ms_per_cost[ranges[r].shape] = sample;
+        else
+          f->second = 0.5 * f->second + 0.5 * sample;
+      }
+    }
     /* advance and retire */
     std::vector<SrvJob *> still;
     for (size_t r = 0; r < s->resident.size(); r++) {

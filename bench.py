@@ -26,6 +26,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The served stream launches one kernel per register layout each round, on
+# up to 16 streams; HIP runs them at once only on as many hardware queues
+# as the process asks for (read when HIP initialises, 4 by default).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 from bbscite_amd import _capi as B  # noqa: E402
 
@@ -233,7 +237,7 @@ def serving_bench(n_jobs):
     jobs = config5_jobs(n_jobs)
     lat = []
     arrival = {}
-    with B.Server(B.server_config(hbm_budget_bytes=4 << 30)) as srv:
+    with B.Server(B.server_config(hbm_budget_bytes=4 << 30, n_streams=16)) as srv:
         t0 = srv.now()
         nxt = 0
         done = 0

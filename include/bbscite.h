@@ -259,16 +259,25 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
  * concurrent-wave capacity free up. Every random draw is keyed by
  * (seed, job_tag, chain, step), so a job's result is the same alone
  * (bbs_create + bbs_run with job_tag 0) and in any mix. */
+#define BBS_MAX_STREAMS 16
 typedef struct bbs_server_config {
   uint64_t hbm_budget_bytes;    /* device memory for resident jobs' state  */
-  uint32_t round_cost;          /* work per round and chain in step-cost
-                                   units (about one per VALU instruction);
-                                   a job advances round_cost / its step
-                                   cost steps per round (at least 1)       */
+  uint32_t round_cost;          /* first round of a layout: a job advances
+                                   round_cost / its step cost steps (the
+                                   cost model counts about one unit per
+                                   VALU instruction of a full rescoring)   */
+  float round_ms;               /* later rounds: steps sized so that each
+                                   job's slice takes about round_ms, from
+                                   the measured time of its layout's last
+                                   launch                                  */
   uint32_t max_resident_chains; /* 0: as many chain waves as the device
                                    runs at once (LDS and registers)        */
   uint32_t waves_per_group;     /* chain waves per workgroup, 1..4          */
-  uint32_t n_streams;           /* 1..4                                     */
+  uint32_t n_streams;           /* 1..BBS_MAX_STREAMS: each round launches
+                                   one kernel per register layout, spread
+                                   over these streams; they run at once
+                                   only on as many hardware queues as HIP
+                                   gives the process (GPU_MAX_HW_QUEUES)   */
 } bbs_server_config;
 
 typedef struct bbs_job_result {
