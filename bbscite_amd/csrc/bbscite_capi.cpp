@@ -1189,12 +1189,15 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   return BBS_OK;
 }
 
-/* LDS floats of each wave's incremental slab for a range, or 0 for full
- * rescoring: incremental wherever the range's workgroups fit in 64 KB. */
-static uint32_t range_wave_floats(const bbs_batch *b, const bbs_batch::Range &R) {
-  if (b->cfg.scoring == BBS_SCORING_FULL) return 0u;
-  const uint32_t f = bbs::batch_wave_floats(R.shape, R.max_nodes);
-  return 4ull * f * b->cfg.waves_per_group <= 65536ull ? f : 0u;
+/* Launch shape of a range: each wave's incremental LDS slab (0: full
+ * rescoring) and the waves per workgroup, halved from the configured value
+ * until the workgroup's slabs fit in 64 KB of LDS. */
+static void range_launch(uint32_t shape, uint32_t max_nodes, uint32_t wpg_cfg, bool full,
+                         uint32_t &wave_floats, uint32_t &wpg) {
+  wpg = wpg_cfg;
+  wave_floats = full ? 0u : bbs::batch_wave_floats(shape, max_nodes);
+  while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > 65536ull) wpg /= 2u;
+  if (4ull * wave_floats * wpg > 65536ull) wave_floats = 0u;
 }
 
 int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
@@ -1241,8 +1244,10 @@ int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
   for (size_t r = 0; r < b->ranges.size() && e == hipSuccess; r++) {
     const auto &R = b->ranges[r];
     hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
-    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, b->cfg.waves_per_group,
-                                range_wave_floats(b, R), st);
+    uint32_t f = 0u, wpg = 1u;
+    range_launch(R.shape, R.max_nodes, b->cfg.waves_per_group,
+                 b->cfg.scoring == BBS_SCORING_FULL, f, wpg);
+    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, wpg, f, st);
   }
   for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++) {
     e = hipEventRecord(b->side_done[i], b->side[i]);
@@ -2028,10 +2033,10 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
     }
     for (size_t r = 0; r < ranges.size(); r++) {
       const R &rg = ranges[r];
-      uint32_t f = bbs::batch_wave_floats(rg.shape, rg.max_nodes);
-      if (4ull * f * wpg > 65536ull) f = 0u;
+      uint32_t f = 0u, wpg_r = 1u;
+      range_launch(rg.shape, rg.max_nodes, wpg, false, f, wpg_r);
       SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r], s->stream[r % ns]));
-      SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg, f,
+      SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg_r, f,
                                           s->stream[r % ns]));
       SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r + 1u], s->stream[r % ns]));
     }

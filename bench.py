@@ -41,6 +41,10 @@ W32 = (G + 31) // 32
 ALG_INT_OPS = N * C * (4 * W32 + 2)  # 126,350 at 94 x 95
 # gfx950 int32 VALU peak: 256 CUs x 64 lanes x 2.4 GHz.
 P_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
+# HBM bytes of one 100,000-step chain launch at 94 x 95 with 1,024 chains,
+# from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors a
+# plain 1 GiB device copy gives those counters: 1,928 KiB x 2 + 6,304 KiB.
+TRAFFIC_100K_WINDOW = (1928 * 2 + 6304) * 1024
 FPGA_KSTEPS = 2938.25  # reference's published chain steps/s at 95 x 94 (kilo)
 
 
@@ -387,7 +391,7 @@ def main():
             "frac": achieved / peak,
             "peak_nominal": P_INT32_TOPS,
             "peak_measured": peak_ops / 1e12,
-            "traffic": None,
+            "traffic": TRAFFIC_100K_WINDOW if S == 100_000 and a.chains == 1024 else None,
             "alg_int_ops_per_step": ALG_INT_OPS,
             "kernel": chain_kernel_name(G, C),
         },
