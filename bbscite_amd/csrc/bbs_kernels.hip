@@ -221,8 +221,25 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     }
   }
 
+  /* the Philox blocks of 64 consecutive steps, one step per lane,
+   * computed on the vector unit; each step reads its 8 words by readlane */
+  uint32_t rbase = a.step_begin;
+  bbs_u32x4 ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
+  bbs_u32x4 rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
   for (uint32_t step = a.step_begin; step < a.step_end; step++) {
-    const bbs_raw_move raw = bbs_draw_move(a.seed, a.job, a.chain, step, N, a.probs);
+    if (step - rbase == 64u) {
+      rbase = step;
+      ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
+      rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+    }
+    bbs_u32x4 wa, wb;
+    const uint32_t sl = step - rbase;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      wa.x[i] = readlane32(ra.x[i], sl);
+      wb.x[i] = readlane32(rb.x[i], sl);
+    }
+    const bbs_raw_move raw = bbs_move_from_blocks(wa, wb, N, a.probs);
     MoveCtx<NS> m;
     m.type = raw.move_type;
     uint32_t v = raw.raw_v, w = raw.raw_w;

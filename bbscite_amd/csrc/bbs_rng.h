@@ -290,11 +290,8 @@ BBS_HD uint32_t bbs_move_type_of(float u, bbs_move_probs p) {
  * u_accept). The normal is evaluated only for beta-change moves: the jump
  * does not influence any other move (src/MutationTree.hpp:128-130).
  */
-BBS_HD bbs_raw_move bbs_draw_move(uint64_t seed, uint32_t job, uint32_t chain,
-                                  uint32_t step, uint32_t n_nodes,
-                                  bbs_move_probs p) {
-  bbs_u32x4 a = bbs_philox_block(seed, job, chain, step, BBS_PURPOSE_MOVE_A);
-  bbs_u32x4 b = bbs_philox_block(seed, job, chain, step, BBS_PURPOSE_MOVE_B);
+BBS_HD bbs_raw_move bbs_move_from_blocks(bbs_u32x4 a, bbs_u32x4 b, uint32_t n_nodes,
+                                         bbs_move_probs p) {
   bbs_raw_move m;
   m.move_type = bbs_move_type_of(bbs_u01f(a.x[0]), p);
   m.raw_v = bbs_uint_below(a.x[1], n_nodes - 1u);
@@ -310,6 +307,14 @@ BBS_HD bbs_raw_move bbs_draw_move(uint64_t seed, uint32_t job, uint32_t chain,
   }
   m.acceptance_level = bbs_u01f(b.x[3]);
   return m;
+}
+
+BBS_HD bbs_raw_move bbs_draw_move(uint64_t seed, uint32_t job, uint32_t chain,
+                                  uint32_t step, uint32_t n_nodes,
+                                  bbs_move_probs p) {
+  return bbs_move_from_blocks(bbs_philox_block(seed, job, chain, step, BBS_PURPOSE_MOVE_A),
+                              bbs_philox_block(seed, job, chain, step, BBS_PURPOSE_MOVE_B),
+                              n_nodes, p);
 }
 
 #endif /* BBS_RNG_H */
