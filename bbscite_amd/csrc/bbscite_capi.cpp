@@ -878,9 +878,9 @@ int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
 struct bbs_batch {
   int device = -1;
   hipStream_t stream = nullptr;           /* stream 0 */
-  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipStream_t side[BBS_MAX_STREAMS - 1] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t side_done[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t side_done[BBS_MAX_STREAMS - 1] = {};
   std::vector<bbs::BatchJob> jobs;
   std::vector<uint32_t> items;
   struct Range {
@@ -1078,7 +1078,7 @@ void bbs_batch_destroy(bbs_batch *b) {
   for (DevBuf *d : bufs) d->release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
-  for (int i = 0; i < 3; i++) {
+  for (int i = 0; i < BBS_MAX_STREAMS - 1; i++) {
     if (b->side[i]) (void)hipStreamSynchronize(b->side[i]);
     if (b->side_done[i]) (void)hipEventDestroy(b->side_done[i]);
     if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
@@ -1108,7 +1108,7 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   else
     bbs_batch_config_default(&b->cfg);
   if (b->cfg.waves_per_group < 1u || b->cfg.waves_per_group > 4u ||
-      b->cfg.n_streams < 1u || b->cfg.n_streams > 4u ||
+      b->cfg.n_streams < 1u || b->cfg.n_streams > (uint32_t)BBS_MAX_STREAMS ||
       (b->cfg.scoring != BBS_SCORING_AUTO && b->cfg.scoring != BBS_SCORING_FULL)) {
     bbs_batch_destroy(b);
     return bfail(nullptr, BBS_ERR_ARG, "invalid batch configuration");
@@ -1320,7 +1320,7 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
   for (auto &j : probe)
     if (j.params.n_steps > probe_steps) j.params.n_steps = probe_steps;
   const uint32_t wpgs[3] = {1u, 2u, 4u};
-  const uint32_t streams[3] = {1u, 2u, 4u};
+  const uint32_t streams[3] = {1u, 4u, 16u};
   float best_t = 1e30f;
   bool found = false;
   for (uint32_t order = 0; order < 2; order++) {

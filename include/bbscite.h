@@ -219,9 +219,13 @@ typedef struct bbs_job {
 } bbs_job;
 
 /* Launch shape (chosen by bbs_batch_autotune). */
+#define BBS_MAX_STREAMS 16
 typedef struct bbs_batch_config {
   uint32_t waves_per_group; /* chains per workgroup: 1, 2 or 4          */
-  uint32_t n_streams;       /* streams the layouts are spread over, 1-4 */
+  uint32_t n_streams;       /* streams the layouts are spread over,
+                               1..BBS_MAX_STREAMS (they overlap on as
+                               many hardware queues as GPU_MAX_HW_QUEUES
+                               gives the process)                       */
   uint32_t order;           /* 0: largest work first; 1: job order      */
   uint32_t persistent;      /* 1: persistent graph-captured kernel;
                                switched off, bbs_batch_run rejects it   */
@@ -259,7 +263,6 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
  * concurrent-wave capacity free up. Every random draw is keyed by
  * (seed, job_tag, chain, step), so a job's result is the same alone
  * (bbs_create + bbs_run with job_tag 0) and in any mix. */
-#define BBS_MAX_STREAMS 16
 typedef struct bbs_server_config {
   uint64_t hbm_budget_bytes;    /* device memory for resident jobs' state  */
   uint32_t round_cost;          /* first round of a layout: a job advances

@@ -412,7 +412,7 @@ def test_batched_items_configs_agree_and_autotune():
     jobs = _mix(6, 77, 60)
     ref = None
     for wpg, nst, order, sc in ((1, 1, 0, "auto"), (4, 2, 1, "auto"), (2, 4, 0, "full"),
-                                (4, 1, 0, "full")):
+                                (4, 1, 0, "full"), (2, 16, 0, "auto")):
         with B.Batch(jobs, B.batch_config(wpg, nst, order, scoring=sc)) as b:
             b.run()
             res = [b.chain_results(j)[2] for j in range(len(jobs))]
@@ -422,4 +422,4 @@ def test_batched_items_configs_agree_and_autotune():
             for x, y in zip(ref, res):
                 assert np.array_equal(_bits(x), _bits(y))
     cfg, ms = B.batch_autotune(jobs, probe_steps=20)
-    assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 2, 4) and ms > 0
+    assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 4, 16) and ms > 0
