@@ -107,6 +107,7 @@ class BatchConfig(C.Structure):
         ("order", C.c_uint32),
         ("persistent", C.c_uint32),
         ("scoring", C.c_uint32),
+        ("graph", C.c_uint32),
     ]
 
 
@@ -550,10 +551,13 @@ def _bcheck(rc, b=None):
         raise BbsError(rc, msg.decode() if msg else "")
 
 
-def batch_config(waves_per_group=None, n_streams=None, order=None, scoring=None):
-    """scoring: None/"auto" (incremental where the tables fit) or "full"."""
+def batch_config(waves_per_group=None, n_streams=None, order=None, scoring=None, graph=None):
+    """scoring: None/"auto" (incremental where the tables fit) or "full";
+    graph: capture the run in a hipGraph on the first run, replay later."""
     c = BatchConfig()
     lib().bbs_batch_config_default(C.byref(c))
+    if graph is not None:
+        c.graph = 1 if graph else 0
     if scoring is not None:
         c.scoring = {"auto": 0, "full": 1}[scoring]
     if waves_per_group is not None:
@@ -574,7 +578,8 @@ def batch_autotune(jobs, probe_steps=2000):
 
 
 class Batch:
-    """Owns one bbs_batch: many datasets in one persistent, graph-captured launch."""
+    """Owns one bbs_batch: many datasets, one launch per register layout
+    (optionally captured in a hipGraph and replayed)."""
 
     def __init__(self, jobs, cfg: BatchConfig | None = None):
         arr, keep = _jobs_array(jobs)

@@ -102,6 +102,8 @@ uint32_t chain_static_lds();
 hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
+/* p[0..n) = value with a kernel (capturable in a hipGraph, unlike a memset) */
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t stream);
 /* items [first, first + count) all have the given shape code; wave_floats
  * 0 for the full scorer, else the LDS floats of each wave's incremental
  * slab (batch_wave_floats of the largest tree in the range) */

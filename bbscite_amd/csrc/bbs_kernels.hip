@@ -1003,6 +1003,19 @@ hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first
   return hipErrorInvalidValue;
 }
 
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t *p, uint32_t value, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t stream) {
+  if (n == 0u) return hipSuccess;
+  uint32_t blocks = (n + 255u) / 256u;
+  if (blocks > 1024u) blocks = 1024u;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, value, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream) {
   uint32_t blocks = a.n_trees < 65536u ? a.n_trees : 65536u;
   if (blocks == 0u) return hipSuccess;

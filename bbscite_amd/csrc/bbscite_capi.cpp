@@ -894,6 +894,9 @@ struct bbs_batch {
   uint64_t total_steps = 0;
   DevBuf d_jobs, d_items, cells, am, dm, best_am, best_dm, beta, score,
       best_beta, best_score, best_step;
+  hipEvent_t ready = nullptr;       /* stream 0 -> side streams */
+  hipGraph_t graph = nullptr;       /* graph mode: the captured run */
+  hipGraphExec_t exec = nullptr;
   bool ran = false;
   std::string err;
 };
@@ -1061,6 +1064,7 @@ void bbs_batch_config_default(bbs_batch_config *cfg) {
   cfg->order = 0u;
   cfg->persistent = 0u;
   cfg->scoring = BBS_SCORING_AUTO;
+  cfg->graph = 0u;
 }
 
 const char *bbs_batch_last_error(const bbs_batch *b) {
@@ -1076,6 +1080,9 @@ void bbs_batch_destroy(bbs_batch *b) {
                     &b->beta,    &b->score,     &b->best_beta, &b->best_score,
                     &b->best_step};
   for (DevBuf *d : bufs) d->release();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->ready) (void)hipEventDestroy(b->ready);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   for (int i = 0; i < BBS_MAX_STREAMS - 1; i++) {
@@ -1109,7 +1116,8 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
     bbs_batch_config_default(&b->cfg);
   if (b->cfg.waves_per_group < 1u || b->cfg.waves_per_group > 4u ||
       b->cfg.n_streams < 1u || b->cfg.n_streams > (uint32_t)BBS_MAX_STREAMS ||
-      (b->cfg.scoring != BBS_SCORING_AUTO && b->cfg.scoring != BBS_SCORING_FULL)) {
+      (b->cfg.scoring != BBS_SCORING_AUTO && b->cfg.scoring != BBS_SCORING_FULL) ||
+      b->cfg.graph > 1u) {
     bbs_batch_destroy(b);
     return bfail(nullptr, BBS_ERR_ARG, "invalid batch configuration");
   }
@@ -1152,6 +1160,7 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ready, hipEventDisableTiming);
   for (uint32_t i = 0; i + 1u < b->cfg.n_streams && e == hipSuccess; i++) {
     e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->side_done[i], hipEventDisableTiming);
@@ -1200,21 +1209,17 @@ static void range_launch(uint32_t shape, uint32_t max_nodes, uint32_t wpg_cfg, b
   if (4ull * wave_floats * wpg > 65536ull) wave_floats = 0u;
 }
 
-int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
-  if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
-  if (b->cfg.persistent)
-    return bfail(b, BBS_ERR_STATE,
-                 "the persistent graph-captured form is switched off: its first GPU "
-                 "test hung and the cause has not been found");
-  BHIP_TRY(b, hipSetDevice(b->device));
+/* Everything one run puts on the streams: initial trees and cleared
+ * maxima on stream 0, then the layout ranges, heaviest first, spread over
+ * the streams, joined back into stream 0. Kernels only (no memset, no host
+ * copies), so it can be captured in a hipGraph. */
+static hipError_t enqueue_run(bbs_batch *b) {
   const bbs::BatchArgs a = batch_args(b);
-  /* initial states: uniform random trees from each job's seed, exactly as
-   * bbs_begin draws them; no maxima yet */
-  std::vector<uint32_t> steps(a.n_items ? b->jobs.back().chain_offset + b->jobs.back().n_chains : 0u,
-                              0xffffffffu);
-  std::vector<float> ninf(steps.size(), -INFINITY);
-  BHIP_TRY(b, hipEventRecord(b->ev0, b->stream));
-  for (const bbs::BatchJob &J : b->jobs) {
+  const uint32_t n_chains =
+      b->jobs.empty() ? 0u : b->jobs.back().chain_offset + b->jobs.back().n_chains;
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < b->jobs.size() && e == hipSuccess; k++) {
+    const bbs::BatchJob &J = b->jobs[k];
     bbs::TreeGenArgs g{};
     g.am = a.am + 2ull * J.row_offset;
     g.dm = a.dm + 2ull * J.row_offset;
@@ -1226,21 +1231,17 @@ int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
     g.purpose = BBS_PURPOSE_INIT_TREE;
     g.n_trees = J.n_chains;
     g.n_nodes = J.n_nodes;
-    BHIP_TRY(b, bbs::launch_random_trees(g, b->stream));
+    e = bbs::launch_random_trees(g, b->stream);
   }
-  BHIP_TRY(b, hipMemcpyAsync(b->best_step.p, steps.data(), steps.size() * 4u,
-                             hipMemcpyHostToDevice, b->stream));
-  BHIP_TRY(b, hipMemcpyAsync(b->best_score.p, ninf.data(), ninf.size() * 4u,
-                             hipMemcpyHostToDevice, b->stream));
-  BHIP_TRY(b, hipMemcpyAsync(b->best_beta.p, ninf.data(), ninf.size() * 4u,
-                             hipMemcpyHostToDevice, b->stream));
-  /* the shape ranges, heaviest first, spread over the streams */
-  hipEvent_t ready = nullptr;
-  BHIP_TRY(b, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-  hipError_t e = hipEventRecord(ready, b->stream);
+  if (e == hipSuccess) e = bbs::launch_fill_u32(a.best_step, 0xffffffffu, n_chains, b->stream);
+  if (e == hipSuccess)
+    e = bbs::launch_fill_u32((uint32_t *)a.best_score, 0xff800000u, n_chains, b->stream);
+  if (e == hipSuccess)
+    e = bbs::launch_fill_u32((uint32_t *)a.best_beta, 0xff800000u, n_chains, b->stream);
   const uint32_t ns = b->cfg.n_streams;
+  if (e == hipSuccess) e = hipEventRecord(b->ready, b->stream);
   for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++)
-    e = hipStreamWaitEvent(b->side[i], ready, 0);
+    e = hipStreamWaitEvent(b->side[i], b->ready, 0);
   for (size_t r = 0; r < b->ranges.size() && e == hipSuccess; r++) {
     const auto &R = b->ranges[r];
     hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
@@ -1253,9 +1254,38 @@ int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
     e = hipEventRecord(b->side_done[i], b->side[i]);
     if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->side_done[i], 0);
   }
+  return e;
+}
+
+int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
+  if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
+  if (b->cfg.persistent)
+    return bfail(b, BBS_ERR_STATE,
+                 "the persistent graph-captured form is switched off: its first GPU "
+                 "test hung and the cause has not been found");
+  BHIP_TRY(b, hipSetDevice(b->device));
+  hipError_t e = hipSuccess;
+  if (b->cfg.graph && !b->exec) {
+    e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t e1 = enqueue_run(b);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      e = e1 != hipSuccess ? e1 : e2;
+      if (e == hipSuccess) {
+        b->graph = g;
+        e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    if (e != hipSuccess)
+      return bfail(b, BBS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  }
+  e = hipEventRecord(b->ev0, b->stream);
+  if (e == hipSuccess) e = b->exec ? hipGraphLaunch(b->exec, b->stream) : enqueue_run(b);
   if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
   if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
-  (void)hipEventDestroy(ready);
   if (e != hipSuccess) return bfail(b, BBS_ERR_HIP, "batch run failed: %s", hipGetErrorString(e));
   float ms = 0.f;
   BHIP_TRY(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));

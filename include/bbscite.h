@@ -231,6 +231,10 @@ typedef struct bbs_batch_config {
                                switched off, bbs_batch_run rejects it   */
   uint32_t scoring;         /* BBS_SCORING_AUTO (incremental where the
                                pair-score tables fit) or _FULL          */
+  uint32_t graph;           /* 1: the first bbs_batch_run captures its
+                               launches (initial trees, resets, every
+                               layout on every stream) in a hipGraph;
+                               later runs replay it                     */
 } bbs_batch_config;
 
 typedef struct bbs_batch bbs_batch;

@@ -423,3 +423,32 @@ def test_batched_items_configs_agree_and_autotune():
                 assert np.array_equal(_bits(x), _bits(y))
     cfg, ms = B.batch_autotune(jobs, probe_steps=20)
     assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 4, 16) and ms > 0
+
+
+def test_batched_graph_replays_equal_eager():
+    """graph = 1: the first run captures initial trees, cleared maxima and
+    every layout launch on four streams in a hipGraph; later runs replay it.
+    Every replay, with eager batched and plain runs in between, gives the
+    eager batched result exactly."""
+    say("[batched graph] eager")
+    jobs = _mix(8, 61, 120)
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4)) as b:
+        b.run()
+        eager = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4, graph=True)) as g:
+        for rep in range(4):
+            say(f"[batched graph] replay {rep}")
+            g.run()
+            got = [(g.chain_results(j), g.best(j)) for j in range(len(jobs))]
+            for j in range(len(jobs)):
+                for x, y in zip(got[j][0], eager[j][0]):
+                    assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                                          _bits(y) if y.dtype == np.float32 else y), (rep, j)
+                assert np.array_equal(got[j][1][0], eager[j][1][0]), (rep, j)
+                assert _bits(got[j][1][3]) == _bits(eager[j][1][3]), (rep, j)
+            # eager work between replays: another batch and a plain run
+            with B.Batch(jobs[:2], B.batch_config()) as other:
+                other.run()
+            p, mut, kno, G = jobs[rep]
+            with B.Context(p, mut, kno, G) as ctx:
+                ctx.run()
