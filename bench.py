@@ -177,7 +177,9 @@ def config4_jobs(steps, chains=64, n_datasets=256):
 
 def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
     """Config 4: the 256-dataset uneven mix, plain launches vs the batched
-    persistent graph. Plain launches run one dataset each, serially."""
+    per-layout launches. Plain launches run one dataset each, serially. At
+    the plain step count the batch is also timed as an eager run and as a
+    replay of its hipGraph capture."""
     jobs_plain = config4_jobs(steps_plain)
     t_plain = 0.0
     bests_plain = []
@@ -193,6 +195,14 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
         b.run()
         same = all(np.float32(b.best(j)[3]).tobytes() == np.float32(bests_plain[j]).tobytes()
                    for j in range(len(jobs_plain)))
+        eager_short = min(b.run() for _ in range(3))
+    # the same short batch captured once in a hipGraph and replayed
+    gcfg = B.batch_config(cfg.waves_per_group, cfg.n_streams, cfg.order, graph=True)
+    with B.Batch(jobs_plain, gcfg) as g:
+        g.run()
+        graph_short = min(g.run() for _ in range(3))
+        same_graph = all(np.float32(g.best(j)[3]).tobytes() == np.float32(bests_plain[j]).tobytes()
+                         for j in range(len(jobs_plain)))
     jobs = config4_jobs(steps_batched)
     with B.Batch(jobs, cfg) as b:
         ms = b.run()
@@ -207,6 +217,9 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
         "speedup": rate_b / rate_p,
         "batched_ms": ms,
         "best_ll_identical_to_plain": same,
+        "short_run": {"steps": steps_plain, "eager_ms": eager_short,
+                      "graph_replay_ms": graph_short,
+                      "graph_best_ll_identical_to_plain": same_graph},
         "autotuned": {"waves_per_group": cfg.waves_per_group,
                       "n_streams": cfg.n_streams, "order": cfg.order},
     }
