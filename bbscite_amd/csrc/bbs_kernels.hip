@@ -314,8 +314,11 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       store_rows<NS>(pdm, N, lds);
       prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
     } else {
+      /* beta moves (or full scoring): with INCR the proposal's pair scores
+       * go straight into the table, since most beta moves are accepted; a
+       * rejection rebuilds the table from the current beta */
       prop_raw = score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
-                                              unused11);
+                                              unused11, INCR ? tab : TableView{nullptr, 0u});
     }
     const float prop_score = prop_raw + prop_prior;
 
@@ -353,15 +356,20 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
           bdm[k][wd] = pdm[k][wd];
         }
     }
+    if (INCR && m.type == BBS_MOVE_CHANGE_BETA && !accept) {
+      /* the table holds the rejected beta's scores: back to the current
+       * beta (the rows in lds.rows are the current tree's) */
+      float best[2];
+      bool n0[2];
+      node_maxima<NS, CS, W, false>(cells, N, L, lds, tab, best, n0, unused01, unused11);
+      wave_lds_sync();
+    }
     if (accept) {
-      if (INCR) {
+      if (INCR && m.type != BBS_MOVE_CHANGE_BETA) {
         /* the table follows the accepted tree (its rows are in lds.rows) */
-        float R[2], best[2];
+        float R[2];
         bool n0[2];
-        if (m.type == BBS_MOVE_CHANGE_BETA)
-          node_maxima<NS, CS, W, false>(cells, N, Lp, lds, tab, best, n0, unused01, unused11);
-        else
-          changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
+        changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
         wave_lds_sync();
       }
       cur_score = prop_score;
