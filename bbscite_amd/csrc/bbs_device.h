@@ -512,7 +512,10 @@ __device__ __forceinline__ uint32_t pop_node(uint64_t (&m)[NS]) {
 
 /* Pair scores of the nodes in `changed` (rows in S.rows), two nodes at a
  * time so that the paired half slot serves both. WRITE stores them in the
- * table; otherwise their per-cell maximum goes to R (node 0 flags NaN). */
+ * table; otherwise their per-cell maximum goes to R, and nan0 flags a NaN
+ * score of node 0 when node 0 is changed. The rows of the next pair are read
+ * while the current pair is scored, alternating between two row buffers (at
+ * the end one pair of rows is read and not used). */
 template <int NS, int CS, int W, bool WRITE>
 __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
                                               const LogProbs &L, WaveLds &S, TableView tab,
@@ -525,47 +528,94 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
   uint64_t m[NS];
 #pragma unroll
   for (int k = 0; k < NS; k++) m[k] = changed[k];
-  bool more = false;
+  auto nonempty = [&]() {
+    bool r = false;
 #pragma unroll
-  for (int k = 0; k < NS; k++) more |= m[k] != 0ull;
-  while (more) {
-    const uint32_t a = pop_node<NS>(m);
-    bool more_b = false;
-#pragma unroll
-    for (int k = 0; k < NS; k++) more_b |= m[k] != 0ull;
-    const uint32_t b = more_b ? pop_node<NS>(m) : a;
-    const uint4 Aa = S.rows[a], Ab = S.rows[b];
-    const uint32_t h = hi_half ? b : a;
-    const uint4 Ah = S.rows[h];
+    for (int k = 0; k < NS; k++) r |= m[k] != 0ull;
+    return r;
+  };
+  if (!nonempty()) return;
+  struct Pair {
+    uint32_t a, b;
+    uint4 Aa, Ab, Ah;
+  };
+  /* the next pair (or the last one again) and its rows; the empty asm
+   * after the reads passes the current rows through, so the current pair's
+   * arithmetic cannot be scheduled ahead of the next pair's reads */
+  auto fetch = [&](Pair &cur, Pair &nx) {
+    nx.a = cur.a;
+    nx.b = cur.b;
+    if (nonempty()) {
+      nx.a = pop_node<NS>(m);
+      nx.b = nonempty() ? pop_node<NS>(m) : nx.a;
+    }
+    nx.Aa = S.rows[nx.a];
+    nx.Ab = S.rows[nx.b];
+    nx.Ah = S.rows[hi_half ? nx.b : nx.a];
+    asm volatile(""
+                 : "+v"(cur.Aa.x), "+v"(cur.Aa.y), "+v"(cur.Aa.z), "+v"(cur.Aa.w),
+                   "+v"(cur.Ab.x), "+v"(cur.Ab.y), "+v"(cur.Ab.z), "+v"(cur.Ab.w),
+                   "+v"(cur.Ah.x), "+v"(cur.Ah.y), "+v"(cur.Ah.z), "+v"(cur.Ah.w)
+                 :
+                 : "memory");
+  };
+  auto score = [&](const Pair &p) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
       uint32_t n01, n11;
-      pair_counts<W>(c.km[j], c.knm[j], Aa, n01, n11);
+      pair_counts<W>(c.km[j], c.knm[j], p.Aa, n01, n11);
       const float sa = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      pair_counts<W>(c.km[j], c.knm[j], Ab, n01, n11);
+      pair_counts<W>(c.km[j], c.knm[j], p.Ab, n01, n11);
       const float sb = pair_score(n01, n11, c.fa[j], c.fb[j], L);
       if (WRITE) {
-        trow0[64u * (uint32_t)j * tab.stride + a] = sa;
-        trow0[64u * (uint32_t)j * tab.stride + b] = sb;
+        trow0[64u * (uint32_t)j * tab.stride + p.a] = sa;
+        trow0[64u * (uint32_t)j * tab.stride + p.b] = sb;
       } else {
         R[j] = vmax(vmax(R[j], sa), sb);
-        if (a == 0u) nan0[j] = sa != sa;
       }
     }
     if constexpr (CS == 3) {
       uint32_t n01, n11;
-      pair_counts<W>(c.km[1], c.knm[1], Ah, n01, n11);
+      pair_counts<W>(c.km[1], c.knm[1], p.Ah, n01, n11);
       const float sh = pair_score(n01, n11, c.fa[1], c.fb[1], L);
-      if (WRITE) {
-        trowh[h] = sh;
-      } else {
+      if (WRITE)
+        trowh[hi_half ? p.b : p.a] = sh;
+      else
         R[1] = vmax(R[1], sh);
-        if (a == 0u && hi_half == 0u) nan0[1] = sh != sh;
-      }
     }
-    more = false;
+  };
+  Pair P, Q;
+  P.a = pop_node<NS>(m);
+  P.b = nonempty() ? pop_node<NS>(m) : P.a;
+  P.Aa = S.rows[P.a];
+  P.Ab = S.rows[P.b];
+  P.Ah = S.rows[hi_half ? P.b : P.a];
+  while (true) {
+    bool more = nonempty();
+    fetch(P, Q);
+    score(P);
+    if (!more) break;
+    more = nonempty();
+    fetch(Q, P);
+    score(Q);
+    if (!more) break;
+  }
+  if (!WRITE && (changed[0] & 1ull) != 0ull) {
+    /* node 0 is changed: its new score decides the NaN flags */
+    const uint4 A0 = S.rows[0];
 #pragma unroll
-    for (int k = 0; k < NS; k++) more |= m[k] != 0ull;
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      uint32_t n01, n11;
+      pair_counts<W>(c.km[j], c.knm[j], A0, n01, n11);
+      const float s0 = pair_score(n01, n11, c.fa[j], c.fb[j], L);
+      nan0[j] = s0 != s0;
+    }
+    if constexpr (CS == 3) {
+      uint32_t n01, n11;
+      pair_counts<W>(c.km[1], c.knm[1], A0, n01, n11);
+      const float s0 = pair_score(n01, n11, c.fa[1], c.fb[1], L);
+      nan0[1] = hi_half == 0u && s0 != s0;
+    }
   }
 }
 
