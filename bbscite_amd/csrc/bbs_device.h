@@ -150,6 +150,12 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
   return r;
 }
 
+/* An empty asm that "rewrites" v: code using v after it cannot be
+ * scheduled before it (volatile asm statements keep their order). */
+__device__ __forceinline__ void pin4(float4 &v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+
 /* IEEE maximum of two arithmetic results as one v_max_f32: both operands
  * are quiet (never signalling) NaNs at worst, so no canonicalisation is
  * needed; a NaN operand yields the other operand, as fmaxf. */
@@ -656,7 +662,10 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
   struct Grp {
     float4 x[2][2], q[2], xh, qhv;
   };
-  auto rd = [&](uint32_t g, Grp &b) { /* g: first 4-node group, even */
+  /* reads groups g, g + 1 (g even) into b; the empty asm after them passes
+   * the other buffer `o` through, so its maxima cannot be scheduled ahead of
+   * these reads */
+  auto rd = [&](uint32_t g, Grp &b, Grp &o) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
       b.x[j][0] = *reinterpret_cast<const float4 *>(r0 + 64u * (uint32_t)j * tab.stride + 4u * g);
@@ -670,6 +679,17 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
       b.qhv = *reinterpret_cast<const float4 *>(qh + 2u * g);
     }
     asm volatile("" ::: "memory");
+#pragma unroll
+    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+      pin4(o.x[j][0]);
+      pin4(o.x[j][1]);
+    }
+    pin4(o.q[0]);
+    pin4(o.q[1]);
+    if constexpr (CS == 3) {
+      pin4(o.xh);
+      pin4(o.qhv);
+    }
   };
   auto mx4 = [&](float &u, const float4 &x, const float4 &q) {
     u = vmax(u, x.x + q.x);
@@ -686,14 +706,30 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
     if constexpr (CS == 3) mx4(U[1], b.xh, b.qhv);
   };
   const uint32_t g4 = (n_nodes + 3u) >> 2; /* 4-node groups of the full slots */
+  /* The reads of the next step are issued unconditionally (past the end
+   * they repeat the last step's groups and are not used): with the same
+   * number of reads in flight on every path, each wait is for exactly the
+   * reads it needs. */
+  const uint32_t gl = (g4 - 1u) & ~1u;
   Grp A, B;
-  rd(0u, A);
-  for (uint32_t g = 0; g < g4; g += 4) {
-    if (g + 2u < g4) rd(g + 2u, B);
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    A.x[j][0] = A.x[j][1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A.q[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  A.xh = A.qhv = make_float4(0.f, 0.f, 0.f, 0.f);
+  B = A;
+  rd(0u, A, B);
+  /* at most 64 NS nodes: straight-line code (exact waits, no loop merge) */
+#pragma unroll
+  for (int it = 0; it < NS * 4; it++) {
+    const uint32_t g = 4u * (uint32_t)it;
+    rd(g + 2u < g4 ? g + 2u : gl, B, A);
     use(A);
     if (g + 2u >= g4) break;
-    if (g + 4u < g4) rd(g + 4u, A);
+    rd(g + 4u < g4 ? g + 4u : gl, A, B);
     use(B);
+    if (g + 4u >= g4) break;
   }
 }
 
