@@ -155,6 +155,9 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {
 __device__ __forceinline__ void pin4(float4 &v) {
   asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
 }
+__device__ __forceinline__ void pin_row(uint4 &v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
 
 /* IEEE maximum of two arithmetic results as one v_max_f32: both operands
  * are quiet (never signalling) NaNs at worst, so no canonicalisation is
@@ -398,10 +401,12 @@ __device__ __forceinline__ float finish_sum(const float (&best)[2], const bool (
 
 /* Node loop of the full scorer over the rows in S.rows: per cell the
  * maximum over nodes (node 0 seeds it, later nodes replace it only when
- * strictly greater, src/TreeScorer.hpp:185-194). With a table (tab.t not
- * null), every pair score is also stored. With one wave per SIMD nothing
- * else hides the LDS latency, so rows are read one node pair ahead. */
-template <int NS, int CS, int W, bool COUNTS>
+ * strictly greater, src/TreeScorer.hpp:185-194). With TAB every pair score is
+ * also stored in the table. With one wave per SIMD nothing else hides the LDS
+ * latency, so rows are read one node pair ahead; past the last node the
+ * clamped index repeats node n - 1, which rewrites its own score and cannot
+ * change a maximum (nor, being equal, the first maximal node). */
+template <int NS, int CS, int W, bool COUNTS, bool TAB>
 __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_nodes,
                                             const LogProbs &L, WaveLds &S, TableView tab,
                                             float (&best)[2], bool (&nan0)[2],
@@ -409,8 +414,8 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
   static_assert(!COUNTS || CS != 3, "counts need full cell slots");
   constexpr int SL = Slots<CS>::n;
   const uint32_t lane = lane_id();
-  float *const trow0 = tab.t ? tab.t + lane * tab.stride : nullptr;
-  float *const trowh = tab.t ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
+  float *const trow0 = TAB ? tab.t + lane * tab.stride : nullptr;
+  float *const trowh = TAB ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
   nan0[0] = nan0[1] = false;
   /* one node for the full slots */
   auto full = [&](const uint4 &A, uint32_t node, bool first) {
@@ -419,7 +424,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
       uint32_t n01, n11;
       pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
       const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      if (trow0) trow0[64u * (uint32_t)j * tab.stride + node] = s;
+      if (TAB) trow0[64u * (uint32_t)j * tab.stride + node] = s;
       if (first) {
         best[j] = s;
         nan0[j] = s != s;
@@ -444,7 +449,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
     constexpr int H = CS == 3 ? 1 : 0;
     pair_counts<W>(c.km[H], c.knm[H], A, n01, n11);
     const float s = pair_score(n01, n11, c.fa[H], c.fb[H], L);
-    if (trowh) trowh[node] = s;
+    if (TAB) trowh[node] = s;
     if (first) {
       best[1] = s;
       nan0[1] = lane < 32u && s != s;
@@ -461,23 +466,30 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
     if (n_nodes > 1u) full(S.rows[1], 1u, false);
     uint4 A0 = S.rows[clampn(2u)], A1 = S.rows[clampn(3u)], Ah = S.rows[clampn(2u + hi_half)];
     for (uint32_t v = 2; v < n_nodes; v += 2) {
-      const uint4 B0 = A0, B1 = A1, Bh = Ah;
+      uint4 B0 = A0, B1 = A1, Bh = Ah;
       A0 = S.rows[clampn(v + 2u)];
       A1 = S.rows[clampn(v + 3u)];
       Ah = S.rows[clampn(v + 2u + hi_half)];
+      asm volatile("" ::: "memory");
+      pin_row(B0);
+      pin_row(B1);
+      pin_row(Bh);
       full(B0, v, false);
       half(Bh, clampn(v + hi_half), false);
-      if (v + 1u < n_nodes) full(B1, v + 1u, false);
+      full(B1, clampn(v + 1u), false);
     }
   } else {
     full(S.rows[0], 0u, true);
     uint4 A1 = S.rows[clampn(1u)], A2 = S.rows[clampn(2u)];
     for (uint32_t v = 1; v < n_nodes; v += 2) {
-      const uint4 B1 = A1, B2 = A2;
+      uint4 B1 = A1, B2 = A2;
       A1 = S.rows[clampn(v + 2u)];
       A2 = S.rows[clampn(v + 3u)];
+      asm volatile("" ::: "memory");
+      pin_row(B1);
+      pin_row(B2);
       full(B1, v, false);
-      if (v + 1u < n_nodes) full(B2, v + 1u, false);
+      full(B2, clampn(v + 1u), false);
     }
   }
 }
@@ -490,7 +502,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
  * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
  * or 2 only), n01/n11 of the first maximal node are returned per slot.
  */
-template <int NS, int CS, int W, bool COUNTS>
+template <int NS, int CS, int W, bool COUNTS, bool TAB = false>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
            uint32_t n_cells, const LogProbs &L, WaveLds &S,
@@ -498,7 +510,7 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
   store_rows<NS>(dm, n_nodes, S);
   float best[2];
   bool nan0[2];
-  node_maxima<NS, CS, W, COUNTS>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
+  node_maxima<NS, CS, W, COUNTS, TAB>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
   return finish_sum<CS>(best, nan0, n_cells, S);
 }
 

@@ -200,7 +200,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   uint32_t unused01[2], unused11[2];
   float cur_score;
   if (a.score_init || INCR) {
-    const float t = score_tree<NS, CS, W, false>(dm, cells, N, a.n_cells, L, lds, unused01,
+    const float t = score_tree<NS, CS, W, false, INCR>(dm, cells, N, a.n_cells, L, lds, unused01,
                                                  unused11, INCR ? tab : TableView{nullptr, 0u});
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
@@ -317,7 +317,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       /* beta moves (or full scoring): with INCR the proposal's pair scores
        * go straight into the table, since most beta moves are accepted; a
        * rejection rebuilds the table from the current beta */
-      prop_raw = score_tree<NS, CS, W, false>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
+      prop_raw = score_tree<NS, CS, W, false, INCR>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
                                               unused11, INCR ? tab : TableView{nullptr, 0u});
     }
     const float prop_score = prop_raw + prop_prior;
@@ -361,7 +361,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
        * beta (the rows in lds.rows are the current tree's) */
       float best[2];
       bool n0[2];
-      node_maxima<NS, CS, W, false>(cells, N, L, lds, tab, best, n0, unused01, unused11);
+      node_maxima<NS, CS, W, false, true>(cells, N, L, lds, tab, best, n0, unused01, unused11);
       wave_lds_sync();
     }
     if (accept) {
