@@ -240,52 +240,78 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       wb.x[i] = readlane32(rb.x[i], sl);
     }
     const bbs_raw_move raw = bbs_move_from_blocks(wa, wb, N, a.probs);
+    /* Realisation (src/MutationTree.hpp:323-363). Without a trace only what
+     * the drawn move type uses is computed (a beta move needs no tree data,
+     * prune and reattach a random non-descendant, a node swap the four rows,
+     * a subtree swap the parents and a random descendant); every value
+     * computed equals the full realisation's. */
     MoveCtx<NS> m;
     m.type = raw.move_type;
+    m.related = false;
+    m.v_target = 0u;
+    m.w_target = 0u;
     uint32_t v = raw.raw_v, w = raw.raw_w;
-    fetch_row<NS>(am, v, m.Dv);
-    if (row_bit<NS>(m.Dv, w)) { /* v is an ancestor of w: swap */
-      uint32_t t = v;
-      v = w;
-      w = t;
-      fetch_row<NS>(am, v, m.Dv);
+    uint32_t nd_v = 0u, nd_w = 0u;
+    uint32_t parent_v = 0u, parent_w = 0u, desc_v = 0u, nondesc_v = 0u;
+    const bool subtrees = m.type == BBS_MOVE_SWAP_SUBTREES;
+    const bool prune = m.type == BBS_MOVE_PRUNE_REATTACH;
+    if (TRACE || m.type != BBS_MOVE_CHANGE_BETA) {
+      /* the four rows at once, then the swap of src/MutationTree.hpp:326-330
+       * (v an ancestor of w) as a selection */
+      uint64_t D0[NS], D1[NS], A0[NS], A1[NS];
+      fetch_row<NS>(am, v, D0);
+      fetch_row<NS>(am, w, D1);
+      fetch_row<NS>(dm, v, A0);
+      fetch_row<NS>(dm, w, A1);
+      const bool sw = row_bit<NS>(D0, w);
+      const uint32_t t = sw ? w : v;
+      w = sw ? v : w;
+      v = t;
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        m.Dv[wd] = sw ? D1[wd] : D0[wd];
+        m.Dw[wd] = sw ? D0[wd] : D1[wd];
+        m.Av[wd] = sw ? A1[wd] : A0[wd];
+        m.Aw[wd] = sw ? A0[wd] : A1[wd];
+        nd_v += (uint32_t)__popcll(m.Dv[wd]);
+      }
+    } else {
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) m.Dv[wd] = m.Dw[wd] = m.Av[wd] = m.Aw[wd] = 0ull;
     }
     m.v = v;
     m.w = w;
-    fetch_row<NS>(am, w, m.Dw);
-    fetch_row<NS>(dm, v, m.Av);
-    fetch_row<NS>(dm, w, m.Aw);
-    m.related = row_bit<NS>(m.Dw, v);
-    const uint32_t parent_v = parent_of<NS>(dm, m.Av, v, N);
-    const uint32_t parent_w = parent_of<NS>(dm, m.Aw, w, N);
-    uint32_t nd_v = 0, nd_w = 0;
+    if (TRACE || subtrees) {
 #pragma unroll
-    for (int wd = 0; wd < NS; wd++) {
-      nd_v += (uint32_t)__popcll(m.Dv[wd]);
-      nd_w += (uint32_t)__popcll(m.Dw[wd]);
+      for (int wd = 0; wd < NS; wd++) nd_w += (uint32_t)__popcll(m.Dw[wd]);
+      m.related = row_bit<NS>(m.Dw, v);
+      parent_w = parent_of<NS>(dm, m.Aw, w, N);
     }
-    const uint32_t nn_v = N - nd_v;
-    const uint32_t i_d = (uint32_t)floorf(raw.raw_descendant_of_v * (float)nd_v);
-    const uint32_t i_n = (uint32_t)floorf(raw.raw_nondescendant_of_v * (float)nn_v);
-    const uint32_t desc_v = select_bit<NS>(m.Dv, i_d, N);
-    uint64_t ndv[NS];
+    if (TRACE || (subtrees && !m.related)) parent_v = parent_of<NS>(dm, m.Av, v, N);
+    if (TRACE || (subtrees && m.related)) {
+      const uint32_t i_d = (uint32_t)floorf(raw.raw_descendant_of_v * (float)nd_v);
+      desc_v = select_bit<NS>(m.Dv, i_d, N);
+    }
+    if (TRACE || prune) {
+      const uint32_t nn_v = N - nd_v;
+      const uint32_t i_n = (uint32_t)floorf(raw.raw_nondescendant_of_v * (float)nn_v);
+      uint64_t ndv[NS];
 #pragma unroll
-    for (int wd = 0; wd < NS; wd++) ndv[wd] = ~m.Dv[wd];
-    const uint32_t nondesc_v = select_bit<NS>(ndv, i_n, N);
+      for (int wd = 0; wd < NS; wd++) ndv[wd] = ~m.Dv[wd];
+      nondesc_v = select_bit<NS>(ndv, i_n, N);
+    }
     float nb = cur_beta + raw.beta_jump;
     if (nb < 0.0f) nb = fabsf(nb);
     if (nb > 1.0f) nb = nb - 2.0f * (nb - 1.0f);
 
-    m.v_target = 0u;
-    m.w_target = 0u;
-    if (m.type == BBS_MOVE_SWAP_SUBTREES) {
+    if (subtrees) {
       m.v_target = parent_w;
       m.w_target = m.related ? desc_v : parent_v;
-    } else if (m.type == BBS_MOVE_PRUNE_REATTACH) {
+    } else if (prune) {
       m.v_target = nondesc_v;
     }
-    fetch_row<NS>(dm, m.v_target, m.At_v);
-    fetch_row<NS>(dm, m.w_target, m.At_w);
+    if (subtrees || prune) fetch_row<NS>(dm, m.v_target, m.At_v);
+    if (subtrees) fetch_row<NS>(dm, m.w_target, m.At_w);
 
     uint64_t pam[NS][NS], pdm[NS][NS];
     apply_move<NS>(m, am, dm, pam, pdm);

@@ -183,6 +183,17 @@ def test_chain_trace_replay(case, mode):
         bs, st, fs, fb = ctx.chain_results()
         am_f, dm_f = ctx.chain_states()
         best_am, best_dm, best_beta, best_score = ctx.best()
+    # the untraced kernel (which realises only what each move type uses)
+    # gives the same chains as the traced one checked below
+    with B.Context(p, mut, kno, G) as ctx:
+        _set_mode(ctx, mode)
+        ctx.run()
+        plain = ctx.chain_results(), ctx.chain_states(), ctx.best()
+    for x, y in zip(plain[0], (bs, st, fs, fb)):
+        assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                              _bits(y) if y.dtype == np.float32 else y)
+    assert np.array_equal(plain[1][0], am_f) and np.array_equal(plain[1][1], dm_f)
+    assert np.array_equal(plain[2][1], best_dm) and _bits(plain[2][3]) == _bits(best_score)
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     for c in range(R):
         res, otr = O.run_chain(p, s, c, c == 0, trace=True)
