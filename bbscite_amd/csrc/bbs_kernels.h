@@ -92,8 +92,11 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
  * incremental chain wave (chain_table_bytes). */
 hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
                         hipStream_t stream);
+/* LDS one workgroup may allocate on gfx950: the whole 160 KB of a CU. */
+constexpr uint32_t kMaxGroupLds = 163840u;
+
 /* LDS bytes of an incremental chain wave (pair-score table, tree rows,
- * sums, node penalties), 0 if the layout has none within 64 KB. */
+ * sums, node penalties), 0 if it exceeds the LDS of one workgroup. */
 uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells);
 /* Static LDS of an incremental chain wave besides chain_table_bytes. */
 uint32_t chain_static_lds();

@@ -906,19 +906,11 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
   if (*ns == 2 && *w < 2) *w = 2;
 }
 
-/* Layouts with an incremental-scoring chain kernel: the pair-score table of
- * the 128-cell layout with two node slots would exceed 64 KB of LDS. */
-template <int NS, int CS>
-constexpr bool has_table() {
-  return CS != 2 || NS == 1;
-}
-
 uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
   int ns, cs, w;
   shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
-  if (cs == 2 && ns == 2) return 0u;
   const uint32_t bytes = 4u * ChainLds::of(n_nodes, cs).total;
-  return bytes <= 65536u ? bytes : 0u;
+  return bytes <= kMaxGroupLds ? bytes : 0u;
 }
 
 uint32_t chain_static_lds() { return 0u; }
@@ -927,18 +919,14 @@ template <int NS, int CS, int W>
 static hipError_t launch_chain_shape(const ChainArgs &a, bool trace, uint32_t table_bytes,
                                      hipStream_t stream) {
   const dim3 grid(a.n_chains), block(64);
-  if constexpr (has_table<NS, CS>()) {
-    if (table_bytes) {
-      if (trace)
-        hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, true>), grid, block, table_bytes,
-                           stream, a);
-      else
-        hipLaunchKernelGGL((chain_kernel<NS, CS, W, false, true>), grid, block, table_bytes,
-                           stream, a);
-      return hipGetLastError();
-    }
-  } else {
-    if (table_bytes) return hipErrorInvalidValue;
+  if (table_bytes) {
+    if (trace)
+      hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, true>), grid, block, table_bytes, stream,
+                         a);
+    else
+      hipLaunchKernelGGL((chain_kernel<NS, CS, W, false, true>), grid, block, table_bytes,
+                         stream, a);
+    return hipGetLastError();
   }
   if (trace)
     hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, false>), grid, block, 0, stream, a);
@@ -1000,8 +988,7 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
 }
 
 uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes) {
-  const int ns = (int)(shape / 100u), cs = (int)(shape / 10u % 10u);
-  if (cs == 2 && ns == 2) return 0u;
+  const int cs = (int)(shape / 10u % 10u);
   return ChainLds::of(max_nodes, cs).total;
 }
 
@@ -1009,14 +996,10 @@ template <int NS, int CS, int W>
 static hipError_t launch_items_shape(const BatchArgs &a, uint32_t first, uint32_t count,
                                      uint32_t wpg, uint32_t wave_floats, hipStream_t stream) {
   const uint32_t groups = (count + wpg - 1u) / wpg;
-  if constexpr (has_table<NS, CS>()) {
-    if (wave_floats) {
-      hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true>), dim3(groups), dim3(64u * wpg),
-                         4u * wave_floats * wpg, stream, a, first, count, wave_floats);
-      return hipGetLastError();
-    }
-  } else {
-    if (wave_floats) return hipErrorInvalidValue;
+  if (wave_floats) {
+    hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true>), dim3(groups), dim3(64u * wpg),
+                       4u * wave_floats * wpg, stream, a, first, count, wave_floats);
+    return hipGetLastError();
   }
   hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, false>), dim3(groups), dim3(64u * wpg), 0,
                      stream, a, first, count, 0u);
@@ -1028,7 +1011,7 @@ hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first
                               hipStream_t stream) {
   if (count == 0u) return hipSuccess;
   if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
-  if (4ull * wave_floats * waves_per_group > 65536ull) return hipErrorInvalidValue;
+  if (4ull * wave_floats * waves_per_group > kMaxGroupLds) return hipErrorInvalidValue;
 #define XB(NS_, CS_, W_)                                                       \
   if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_))                        \
     return launch_items_shape<NS_, CS_, W_>(a, first, count, waves_per_group, wave_floats, stream);

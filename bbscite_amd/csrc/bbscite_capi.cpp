@@ -513,7 +513,7 @@ int bbs_set_scoring_mode(bbs_ctx *ctx, int mode) {
     return fail(ctx, BBS_ERR_ARG, "unknown scoring mode %d", mode);
   if (mode == BBS_SCORING_INCREMENTAL && bbs::chain_table_bytes(ctx->n_nodes, ctx->n_cells) == 0u)
     return fail(ctx, BBS_ERR_SIZE,
-                "no incremental scorer for %u genes x %u cells (table over 64 KB of LDS)",
+                "no incremental scorer for %u genes x %u cells (table over the LDS of a workgroup)",
                 ctx->n_genes, ctx->n_cells);
   ctx->scoring_mode = mode;
   return BBS_OK;
@@ -1198,15 +1198,29 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   return BBS_OK;
 }
 
+/* Incremental slab of a layout (floats per wave), or 0 where the table
+ * would cost concurrency: a range of `count` items runs on as many waves at
+ * once with the slabs (LDS-bound, slabs per CU) as without (one per SIMD),
+ * as the plain kernel's automatic rule decides. count = UINT32_MAX asks for
+ * a saturated device (the server). */
+static uint32_t range_slab(uint32_t shape, uint32_t max_nodes, uint32_t count, uint32_t n_cus) {
+  const uint32_t f = bbs::batch_wave_floats(shape, max_nodes);
+  const uint64_t per_cu = bbs::kMaxGroupLds / (4ull * f);
+  const uint64_t cus = n_cus ? n_cus : 1u;
+  const uint64_t with_table = std::min<uint64_t>(count, cus * per_cu);
+  const uint64_t without = std::min<uint64_t>(count, cus * 4u);
+  return per_cu > 0u && with_table >= without ? f : 0u;
+}
+
 /* Launch shape of a range: each wave's incremental LDS slab (0: full
  * rescoring) and the waves per workgroup, halved from the configured value
- * until the workgroup's slabs fit in 64 KB of LDS. */
+ * until the workgroup's slabs fit in the LDS one workgroup may hold. */
 static void range_launch(uint32_t shape, uint32_t max_nodes, uint32_t wpg_cfg, bool full,
-                         uint32_t &wave_floats, uint32_t &wpg) {
+                         uint32_t count, uint32_t n_cus, uint32_t &wave_floats, uint32_t &wpg) {
   wpg = wpg_cfg;
-  wave_floats = full ? 0u : bbs::batch_wave_floats(shape, max_nodes);
-  while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > 65536ull) wpg /= 2u;
-  if (4ull * wave_floats * wpg > 65536ull) wave_floats = 0u;
+  wave_floats = full ? 0u : range_slab(shape, max_nodes, count, n_cus);
+  while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > bbs::kMaxGroupLds) wpg /= 2u;
+  if (4ull * wave_floats * wpg > bbs::kMaxGroupLds) wave_floats = 0u;
 }
 
 /* Everything one run puts on the streams: initial trees and cleared
@@ -1247,7 +1261,7 @@ static hipError_t enqueue_run(bbs_batch *b) {
     hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
     uint32_t f = 0u, wpg = 1u;
     range_launch(R.shape, R.max_nodes, b->cfg.waves_per_group,
-                 b->cfg.scoring == BBS_SCORING_FULL, f, wpg);
+                 b->cfg.scoring == BBS_SCORING_FULL, R.count, b->n_cus, f, wpg);
     e = bbs::launch_batch_items(a, R.shape, R.first, R.count, wpg, f, st);
   }
   for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++) {
@@ -1933,7 +1947,7 @@ int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id) {
   j->n_genes = G;
   j->n_steps = job->params.n_steps;
   j->cost = step_cost(j->J);
-  const uint32_t f = bbs::batch_wave_floats(j->J.shape, j->J.n_nodes);
+  const uint32_t f = range_slab(j->J.shape, j->J.n_nodes, UINT32_MAX, s->n_cus);
   j->lds = f ? 4ull * f : 2560ull;
   j->t_submit = srv_now(s);
   j->id = s->next_id++;
@@ -2064,7 +2078,7 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
     for (size_t r = 0; r < ranges.size(); r++) {
       const R &rg = ranges[r];
       uint32_t f = 0u, wpg_r = 1u;
-      range_launch(rg.shape, rg.max_nodes, wpg, false, f, wpg_r);
+      range_launch(rg.shape, rg.max_nodes, wpg, false, UINT32_MAX, s->n_cus, f, wpg_r);
       SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r], s->stream[r % ns]));
       SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg_r, f,
                                           s->stream[r % ns]));

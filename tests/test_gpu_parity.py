@@ -165,7 +165,7 @@ def _set_mode(ctx, mode):
         ctx.set_scoring_mode(mode)
     except B.BbsError as e:
         if mode == "incremental" and e.code == -2:
-            pytest.skip("no incremental scorer for this size (table over 64 KB of LDS)")
+            pytest.skip("no incremental scorer for this size (table over the LDS of a workgroup)")
         raise
 
 
