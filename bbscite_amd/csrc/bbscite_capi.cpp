@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <thread>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -1652,28 +1653,41 @@ struct SrvJob {
   bool started = false;
 };
 
+/* The resident jobs of one register layout and the slice they run: a lane
+ * relaunches as soon as its own slice ends, on its own stream, so layouts
+ * never wait for each other. */
+struct Lane {
+  uint32_t shape = 0;
+  hipStream_t stream = nullptr;      /* one of the server's streams       */
+  std::vector<SrvJob *> jobs;        /* resident, in admission order      */
+  /* the slice in flight (valid while busy) */
+  bool busy = false;
+  std::vector<SrvJob *> slice_jobs;
+  std::vector<bbs::BatchJob> h_jobs; /* host sources of the uploads stay  */
+  std::vector<uint32_t> h_items;     /* alive until the slice completes   */
+  double slice_work = 0.0;           /* heaviest job's cost x steps       */
+  hipEvent_t ev_start = nullptr, ev_end = nullptr;
+  DevBuf d_jobs, d_items;
+};
+
 } // namespace
 
 struct bbs_server {
   int device = -1;
   bbs_server_config cfg{};
   hipStream_t stream[BBS_MAX_STREAMS] = {};
-  hipEvent_t done_ev[BBS_MAX_STREAMS] = {};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> range_ev; /* start/end per layout range */
+  std::vector<Lane *> lanes;
   std::map<uint32_t, double> ms_per_cost; /* layout -> ms per (step x cost unit) */
   uint32_t n_cus = 0, lds_per_cu = 0;
-  DevBuf am, dm, best_am, best_dm, beta, score, best_beta, best_score, best_step, cells,
-      d_jobs, d_items;
+  DevBuf am, dm, best_am, best_dm, beta, score, best_beta, best_score, best_step, cells;
   Arena rows, chains, cellsA;
   std::deque<SrvJob *> pending;
-  std::vector<SrvJob *> resident;
   std::deque<bbs_job_result> done;
   uint64_t next_id = 1;
   uint64_t lds_cap = 0, lds_used = 0;
-  uint32_t chain_cap = 0, chains_used = 0;
+  uint32_t chain_cap = 0, chains_used = 0, n_resident = 0, n_busy = 0;
   uint64_t steps_total = 0, rounds = 0;
-  double busy_s = 0.0;
+  double busy_s = 0.0, busy_since = 0.0;
   std::chrono::steady_clock::time_point t0;
   std::string err;
 };
@@ -1736,11 +1750,34 @@ void srv_free_job(bbs_server *s, SrvJob *j) {
 
 /* Admit the head of the queue if memory and concurrent-wave capacity allow
  * (an empty server admits any job, so none waits forever). */
+/* The lane of a layout, created on first use: streams are dealt out in
+ * creation order, and the job and item tables are sized for the most a
+ * lane can hold (chain_cap chains), so they never grow under a slice. */
+int srv_lane(bbs_server *s, uint32_t shape, Lane *&out) {
+  for (Lane *l : s->lanes)
+    if (l->shape == shape) {
+      out = l;
+      return BBS_OK;
+    }
+  Lane *l = new Lane();
+  l->shape = shape;
+  l->stream = s->stream[s->lanes.size() % s->cfg.n_streams];
+  s->lanes.push_back(l);
+  SHIP_TRY(s, hipEventCreate(&l->ev_start));
+  SHIP_TRY(s, hipEventCreate(&l->ev_end));
+  int rc;
+  if ((rc = srv_alloc(s, l->d_jobs, (size_t)s->chain_cap * sizeof(bbs::BatchJob))) ||
+      (rc = srv_alloc(s, l->d_items, (size_t)s->chain_cap * 4u)))
+    return rc;
+  out = l;
+  return BBS_OK;
+}
+
 int srv_admit(bbs_server *s, bool &admitted) {
   admitted = false;
   SrvJob *j = s->pending.front();
   bbs::BatchJob &J = j->J;
-  const bool empty = s->resident.empty();
+  const bool empty = s->n_resident == 0u;
   if (!empty && (s->chains_used + J.n_chains > s->chain_cap ||
                  s->lds_used + (uint64_t)J.n_chains * j->lds > s->lds_cap))
     return BBS_OK;
@@ -1761,27 +1798,34 @@ int srv_admit(bbs_server *s, bool &admitted) {
   J.cell_offset = (uint32_t)x;
   s->lds_used += (uint64_t)J.n_chains * j->lds;
   s->chains_used += J.n_chains;
+  Lane *l = nullptr;
+  int rc = srv_lane(s, J.shape, l);
+  if (rc) return rc;
+  /* queued behind the lane's slice in flight, which does not touch it */
   SHIP_TRY(s, hipMemcpyAsync((uint64_t *)s->cells.p + 4ull * x, j->cells.data(),
-                             j->cells.size() * 8u, hipMemcpyHostToDevice, s->stream[0]));
+                             j->cells.size() * 8u, hipMemcpyHostToDevice, l->stream));
   j->t_admit = srv_now(s);
   s->pending.pop_front();
-  s->resident.push_back(j);
+  l->jobs.push_back(j);
+  s->n_resident++;
   admitted = true;
   return BBS_OK;
 }
 
 /* The finished job's result: the chain-major first maximum of its chains
  * (src/Application.hpp:456), as bbs_get_best. */
-int srv_retire(bbs_server *s, SrvJob *j, double t_done) {
+int srv_retire(bbs_server *s, hipStream_t st_, SrvJob *j, double t_done) {
   const bbs::BatchJob &J = j->J;
   std::vector<float> sc(J.n_chains), bb(J.n_chains);
   std::vector<uint32_t> st(J.n_chains);
-  SHIP_TRY(s, hipMemcpy(sc.data(), (float *)s->best_score.p + J.chain_offset,
-                        J.n_chains * 4u, hipMemcpyDeviceToHost));
-  SHIP_TRY(s, hipMemcpy(st.data(), (uint32_t *)s->best_step.p + J.chain_offset,
-                        J.n_chains * 4u, hipMemcpyDeviceToHost));
-  SHIP_TRY(s, hipMemcpy(bb.data(), (float *)s->best_beta.p + J.chain_offset,
-                        J.n_chains * 4u, hipMemcpyDeviceToHost));
+  /* on the (idle) lane stream, so other lanes' slices keep running */
+  SHIP_TRY(s, hipMemcpyAsync(sc.data(), (float *)s->best_score.p + J.chain_offset,
+                             J.n_chains * 4u, hipMemcpyDeviceToHost, st_));
+  SHIP_TRY(s, hipMemcpyAsync(st.data(), (uint32_t *)s->best_step.p + J.chain_offset,
+                             J.n_chains * 4u, hipMemcpyDeviceToHost, st_));
+  SHIP_TRY(s, hipMemcpyAsync(bb.data(), (float *)s->best_beta.p + J.chain_offset,
+                             J.n_chains * 4u, hipMemcpyDeviceToHost, st_));
+  SHIP_TRY(s, hipStreamSynchronize(st_));
   uint32_t best = 0u;
   for (uint32_t c = 1; c < J.n_chains; c++)
     if (st[c] != 0xffffffffu && sc[c] > sc[best]) best = c;
@@ -1800,12 +1844,141 @@ int srv_retire(bbs_server *s, SrvJob *j, double t_done) {
   r.admit_s = j->t_admit;
   r.done_s = t_done;
   const size_t off = (size_t)(J.row_offset + (uint64_t)best * J.n_nodes) * 16u;
-  SHIP_TRY(s, hipMemcpy(r.best_am, (char *)s->best_am.p + off, (size_t)J.n_nodes * 16u,
-                        hipMemcpyDeviceToHost));
-  SHIP_TRY(s, hipMemcpy(r.best_dm, (char *)s->best_dm.p + off, (size_t)J.n_nodes * 16u,
-                        hipMemcpyDeviceToHost));
+  SHIP_TRY(s, hipMemcpyAsync(r.best_am, (char *)s->best_am.p + off, (size_t)J.n_nodes * 16u,
+                             hipMemcpyDeviceToHost, st_));
+  SHIP_TRY(s, hipMemcpyAsync(r.best_dm, (char *)s->best_dm.p + off, (size_t)J.n_nodes * 16u,
+                             hipMemcpyDeviceToHost, st_));
+  SHIP_TRY(s, hipStreamSynchronize(st_));
   s->done.push_back(r);
+  s->n_resident--;
   srv_free_job(s, j);
+  return BBS_OK;
+}
+
+/* Launches the next slice of every resident job of an idle lane: new jobs'
+ * initial trees and cleared maxima first, then one items launch. Each job
+ * advances about round_ms (from the layout's measured ms per cost unit), or
+ * round_cost cost units before the layout has been measured. */
+int srv_lane_launch(bbs_server *s, Lane *l) {
+  const size_t n = l->jobs.size();
+  l->slice_jobs = l->jobs;
+  l->h_jobs.resize(n);
+  l->h_items.clear();
+  l->slice_work = 0.0;
+  auto f = s->ms_per_cost.find(l->shape);
+  std::vector<uint32_t> order(n);
+  for (uint32_t r = 0; r < n; r++) {
+    SrvJob *j = l->jobs[r];
+    bbs::BatchJob J = j->J;
+    const double want = f == s->ms_per_cost.end()
+                            ? (double)s->cfg.round_cost / j->cost
+                            : (double)s->cfg.round_ms / (f->second * j->cost);
+    const uint64_t adv = (uint64_t)std::max(1.0, std::min(want, 4.0e9));
+    J.step_begin = j->steps_done;
+    J.step_end = (uint32_t)std::min<uint64_t>(j->n_steps, j->steps_done + adv);
+    J.score_init = j->started ? 0u : 1u;
+    l->h_jobs[r] = J;
+    l->slice_work = std::max(l->slice_work, j->cost * (double)(J.step_end - J.step_begin));
+    order[r] = r;
+    if (!j->started) {
+      /* initial trees as bbs_begin draws them, no maxima yet */
+      bbs::TreeGenArgs g{};
+      g.am = (uint64_t *)s->am.p + 2ull * J.row_offset;
+      g.dm = (uint64_t *)s->dm.p + 2ull * J.row_offset;
+      g.beta = (float *)s->beta.p + J.chain_offset;
+      g.beta_value = J.beta_mean;
+      g.seed = J.seed;
+      g.job = J.job_tag;
+      g.chain_offset = 0u;
+      g.purpose = BBS_PURPOSE_INIT_TREE;
+      g.n_trees = J.n_chains;
+      g.n_nodes = J.n_nodes;
+      SHIP_TRY(s, bbs::launch_random_trees(g, l->stream));
+      SHIP_TRY(s, bbs::launch_fill_u32((uint32_t *)s->best_step.p + J.chain_offset, 0xffffffffu,
+                                       J.n_chains, l->stream));
+      SHIP_TRY(s, bbs::launch_fill_u32((uint32_t *)s->best_score.p + J.chain_offset,
+                                       0xff800000u, J.n_chains, l->stream));
+      SHIP_TRY(s, bbs::launch_fill_u32((uint32_t *)s->best_beta.p + J.chain_offset,
+                                       0xff800000u, J.n_chains, l->stream));
+    }
+  }
+  /* items: the largest work first */
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const double wx = l->jobs[x]->cost * (double)(l->h_jobs[x].step_end - l->h_jobs[x].step_begin);
+    const double wy = l->jobs[y]->cost * (double)(l->h_jobs[y].step_end - l->h_jobs[y].step_begin);
+    return wx > wy;
+  });
+  for (uint32_t r : order)
+    for (uint32_t c = 0; c < l->h_jobs[r].n_chains; c++) l->h_items.push_back((r << 16) | c);
+  SHIP_TRY(s, hipMemcpyAsync(l->d_jobs.p, l->h_jobs.data(), n * sizeof(bbs::BatchJob),
+                             hipMemcpyHostToDevice, l->stream));
+  SHIP_TRY(s, hipMemcpyAsync(l->d_items.p, l->h_items.data(), l->h_items.size() * 4u,
+                             hipMemcpyHostToDevice, l->stream));
+  bbs::BatchArgs a{};
+  a.jobs = (const bbs::BatchJob *)l->d_jobs.p;
+  a.items = (const uint32_t *)l->d_items.p;
+  a.n_items = (uint32_t)l->h_items.size();
+  a.cells = (const uint64_t *)s->cells.p;
+  a.am = (uint64_t *)s->am.p;
+  a.dm = (uint64_t *)s->dm.p;
+  a.best_am = (uint64_t *)s->best_am.p;
+  a.best_dm = (uint64_t *)s->best_dm.p;
+  a.beta = (float *)s->beta.p;
+  a.score = (float *)s->score.p;
+  a.best_beta = (float *)s->best_beta.p;
+  a.best_score = (float *)s->best_score.p;
+  a.best_step = (uint32_t *)s->best_step.p;
+  uint32_t max_nodes = 0u;
+  for (const auto &J : l->h_jobs) max_nodes = std::max(max_nodes, J.n_nodes);
+  /* table or full scoring for this slice from its own item count: a lane
+   * with few chains (a long job in the tail) takes the faster table even
+   * where a saturated device would not; either gives identical chains */
+  uint32_t fl = 0u, wpg_r = 1u;
+  range_launch(l->shape, max_nodes, s->cfg.waves_per_group, false, a.n_items, s->n_cus, fl,
+               wpg_r);
+  SHIP_TRY(s, hipEventRecord(l->ev_start, l->stream));
+  SHIP_TRY(s, bbs::launch_batch_items(a, l->shape, 0u, a.n_items, wpg_r, fl, l->stream));
+  SHIP_TRY(s, hipEventRecord(l->ev_end, l->stream));
+  l->busy = true;
+  if (s->n_busy++ == 0u) s->busy_since = srv_now(s);
+  s->rounds++;
+  return BBS_OK;
+}
+
+/* A lane's slice has ended: its measured time updates the layout's ms per
+ * cost unit, its jobs advance, and the finished ones retire. */
+int srv_lane_complete(bbs_server *s, Lane *l) {
+  const double t_end = srv_now(s);
+  float ms = 0.f;
+  SHIP_TRY(s, hipEventElapsedTime(&ms, l->ev_start, l->ev_end));
+  if (l->slice_work > 0.0 && ms > 0.f) {
+    const double sample = ms / l->slice_work;
+    auto f = s->ms_per_cost.find(l->shape);
+    if (f == s->ms_per_cost.end())
+      s->ms_per_cost[l->shape] = sample;
+    else
+      f->second = 0.5 * f->second + 0.5 * sample;
+  }
+  l->busy = false;
+  if (--s->n_busy == 0u) s->busy_s += t_end - s->busy_since;
+  std::vector<SrvJob *> still;
+  for (size_t r = 0; r < l->slice_jobs.size(); r++) {
+    SrvJob *j = l->slice_jobs[r];
+    const bbs::BatchJob &J = l->h_jobs[r];
+    s->steps_total += (uint64_t)J.n_chains * (J.step_end - J.step_begin);
+    j->steps_done = J.step_end;
+    j->started = true;
+  }
+  for (SrvJob *j : l->jobs) {
+    if (j->started && j->steps_done >= j->n_steps) {
+      const int rc = srv_retire(s, l->stream, j, t_end);
+      if (rc) return rc;
+    } else {
+      still.push_back(j);
+    }
+  }
+  l->jobs.swap(still);
+  l->slice_jobs.clear();
   return BBS_OK;
 }
 
@@ -1833,18 +2006,19 @@ void bbs_server_destroy(bbs_server *s) {
   for (int i = 0; i < BBS_MAX_STREAMS; i++)
     if (s->stream[i]) (void)hipStreamSynchronize(s->stream[i]);
   for (SrvJob *j : s->pending) delete j;
-  for (SrvJob *j : s->resident) delete j;
-  DevBuf *bufs[] = {&s->am,        &s->dm,         &s->best_am,   &s->best_dm,
-                    &s->beta,      &s->score,      &s->best_beta, &s->best_score,
-                    &s->best_step, &s->cells,      &s->d_jobs,    &s->d_items};
-  for (DevBuf *d : bufs) d->release();
-  for (int i = 0; i < BBS_MAX_STREAMS; i++) {
-    if (s->done_ev[i]) (void)hipEventDestroy(s->done_ev[i]);
-    if (s->stream[i]) (void)hipStreamDestroy(s->stream[i]);
+  for (Lane *l : s->lanes) {
+    for (SrvJob *j : l->jobs) delete j;
+    l->d_jobs.release();
+    l->d_items.release();
+    if (l->ev_start) (void)hipEventDestroy(l->ev_start);
+    if (l->ev_end) (void)hipEventDestroy(l->ev_end);
+    delete l;
   }
-  if (s->ev0) (void)hipEventDestroy(s->ev0);
-  if (s->ev1) (void)hipEventDestroy(s->ev1);
-  for (hipEvent_t e : s->range_ev) (void)hipEventDestroy(e);
+  DevBuf *bufs[] = {&s->am,   &s->dm,    &s->best_am,   &s->best_dm,    &s->beta,
+                    &s->score, &s->best_beta, &s->best_score, &s->best_step, &s->cells};
+  for (DevBuf *d : bufs) d->release();
+  for (int i = 0; i < BBS_MAX_STREAMS; i++)
+    if (s->stream[i]) (void)hipStreamDestroy(s->stream[i]);
   delete s;
 }
 
@@ -1874,12 +2048,8 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
   hipDeviceProp_t prop;
   hipError_t e = hipSetDevice(dev);
   if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
-  for (uint32_t i = 0; i < c.n_streams && e == hipSuccess; i++) {
+  for (uint32_t i = 0; i < c.n_streams && e == hipSuccess; i++)
     e = hipStreamCreateWithFlags(&s->stream[i], hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done_ev[i], hipEventDisableTiming);
-  }
-  if (e == hipSuccess) e = hipEventCreate(&s->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&s->ev1);
   if (e != hipSuccess) {
     bbs_server_destroy(s);
     return sfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
@@ -1888,8 +2058,15 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
   s->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
   /* as many chain waves as run at once: LDS per CU, and two waves per SIMD
    * (a chain wave uses up to 256 VGPRs) */
-  s->lds_cap = (uint64_t)s->n_cus * s->lds_per_cu;
-  s->chain_cap = c.max_resident_chains ? c.max_resident_chains : s->n_cus * 8u;
+  /* default: 16 times the chain waves that run at once (two per SIMD), so
+   * that long jobs are admitted early and share the device rather than
+   * wait behind the queue */
+  s->chain_cap = c.max_resident_chains ? c.max_resident_chains : s->n_cus * 8u * 16u;
+  /* a resident set above what runs at once is queued by the hardware: the
+   * LDS budget grows by the same factor */
+  const uint64_t at_once = (uint64_t)s->n_cus * 8u;
+  const uint64_t over = std::max<uint64_t>(1u, (s->chain_cap + at_once - 1u) / at_once);
+  s->lds_cap = (uint64_t)s->n_cus * s->lds_per_cu * over;
   /* device memory: 64 B per tree row (am, dm and their bests), 20 B per
    * chain, 32 B per cell; rows take 90 % of the budget */
   const uint64_t B = c.hbm_budget_bytes;
@@ -1960,176 +2137,48 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
                     uint32_t *n_done_ready) {
   if (!s) return sfail(nullptr, BBS_ERR_ARG, "server is NULL");
   SHIP_TRY(s, hipSetDevice(s->device));
-  /* admission in arrival order */
-  while (!s->pending.empty()) {
-    bool ok = false;
-    int rc = srv_admit(s, ok);
-    if (rc) return rc;
-    if (!ok) break;
+  bool progressed = false;
+  for (int pass = 0; pass < 2; pass++) {
+    /* lanes whose slice has ended */
+    for (Lane *l : s->lanes) {
+      if (!l->busy) continue;
+      const hipError_t q = hipEventQuery(l->ev_end);
+      if (q == hipErrorNotReady) continue;
+      SHIP_TRY(s, q);
+      const int rc = srv_lane_complete(s, l);
+      if (rc) return rc;
+      progressed = true;
+    }
+    /* admission in arrival order */
+    while (!s->pending.empty()) {
+      bool ok = false;
+      const int rc = srv_admit(s, ok);
+      if (rc) return rc;
+      if (!ok) break;
+    }
+    /* every idle lane with resident jobs runs its next slice */
+    for (Lane *l : s->lanes) {
+      if (l->busy || l->jobs.empty()) continue;
+      const int rc = srv_lane_launch(s, l);
+      if (rc) return rc;
+      progressed = true;
+    }
+    if (progressed || s->n_busy == 0u) break;
+    /* nothing to do until a slice ends: poll every busy lane (HIP has no
+     * wait-for-any), so whichever ends first is relaunched at once */
+    for (bool ended = false; !ended;) {
+      for (Lane *l : s->lanes) {
+        if (!l->busy) continue;
+        const hipError_t q = hipEventQuery(l->ev_end);
+        if (q == hipErrorNotReady) continue;
+        SHIP_TRY(s, q);
+        ended = true;
+        break;
+      }
+      if (!ended) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
   }
-  if (!s->resident.empty()) {
-    const double t_round = srv_now(s);
-    /* this round's windows: equal work per chain in step-cost units */
-    std::vector<bbs::BatchJob> jobs(s->resident.size());
-    std::vector<float> ninf;
-    for (size_t r = 0; r < s->resident.size(); r++) {
-      SrvJob *j = s->resident[r];
-      bbs::BatchJob J = j->J;
-      auto f = s->ms_per_cost.find(j->J.shape);
-      const double want = f == s->ms_per_cost.end()
-                              ? (double)s->cfg.round_cost / j->cost
-                              : (double)s->cfg.round_ms / (f->second * j->cost);
-      const uint64_t adv = (uint64_t)std::max(1.0, std::min(want, 4.0e9));
-      J.step_begin = j->steps_done;
-      J.step_end = (uint32_t)std::min<uint64_t>(j->n_steps, j->steps_done + adv);
-      J.score_init = j->started ? 0u : 1u;
-      jobs[r] = J;
-    }
-    SHIP_TRY(s, hipEventRecord(s->ev0, s->stream[0]));
-    /* newly admitted jobs: initial trees as bbs_begin draws them, no maxima
-     * (the -inf source stays allocated until the round has synchronised) */
-    uint32_t new_chains = 0;
-    for (SrvJob *j : s->resident)
-      if (!j->started) new_chains = std::max(new_chains, j->J.n_chains);
-    ninf.assign(new_chains, -INFINITY);
-    for (SrvJob *j : s->resident) {
-      if (j->started) continue;
-      const bbs::BatchJob &J = j->J;
-      bbs::TreeGenArgs g{};
-      g.am = (uint64_t *)s->am.p + 2ull * J.row_offset;
-      g.dm = (uint64_t *)s->dm.p + 2ull * J.row_offset;
-      g.beta = (float *)s->beta.p + J.chain_offset;
-      g.beta_value = J.beta_mean;
-      g.seed = J.seed;
-      g.job = J.job_tag;
-      g.chain_offset = 0u;
-      g.purpose = BBS_PURPOSE_INIT_TREE;
-      g.n_trees = J.n_chains;
-      g.n_nodes = J.n_nodes;
-      SHIP_TRY(s, bbs::launch_random_trees(g, s->stream[0]));
-      SHIP_TRY(s, hipMemsetAsync((uint32_t *)s->best_step.p + J.chain_offset, 0xff,
-                                 J.n_chains * 4u, s->stream[0]));
-      SHIP_TRY(s, hipMemcpyAsync((float *)s->best_score.p + J.chain_offset, ninf.data(),
-                                 J.n_chains * 4u, hipMemcpyHostToDevice, s->stream[0]));
-      SHIP_TRY(s, hipMemcpyAsync((float *)s->best_beta.p + J.chain_offset, ninf.data(),
-                                 J.n_chains * 4u, hipMemcpyHostToDevice, s->stream[0]));
-    }
-    /* items grouped by register layout, largest work first */
-    std::vector<uint32_t> order(jobs.size());
-    for (uint32_t r = 0; r < order.size(); r++) order[r] = r;
-    auto work = [&](uint32_t r) {
-      return s->resident[r]->cost * (double)(jobs[r].step_end - jobs[r].step_begin);
-    };
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-      if (jobs[x].shape != jobs[y].shape) return jobs[x].shape < jobs[y].shape;
-      return work(x) > work(y);
-    });
-    std::vector<uint32_t> items;
-    struct R {
-      uint32_t shape, first, count, max_nodes;
-      double work;
-    };
-    std::vector<R> ranges;
-    for (size_t i = 0; i < order.size();) {
-      R rg{jobs[order[i]].shape, (uint32_t)items.size(), 0u, 0u, 0.0};
-      for (; i < order.size() && jobs[order[i]].shape == rg.shape; i++) {
-        const uint32_t r = order[i];
-        for (uint32_t c = 0; c < jobs[r].n_chains; c++) items.push_back((r << 16) | c);
-        rg.max_nodes = std::max(rg.max_nodes, jobs[r].n_nodes);
-        rg.work += work(r) * jobs[r].n_chains;
-      }
-      rg.count = (uint32_t)items.size() - rg.first;
-      ranges.push_back(rg);
-    }
-    std::stable_sort(ranges.begin(), ranges.end(),
-                     [](const R &x, const R &y) { return x.work > y.work; });
-    int rc;
-    if ((rc = srv_reserve(s, s->d_jobs, jobs.size() * sizeof(bbs::BatchJob))) ||
-        (rc = srv_reserve(s, s->d_items, items.size() * 4u)))
-      return rc;
-    SHIP_TRY(s, hipMemcpyAsync(s->d_jobs.p, jobs.data(), jobs.size() * sizeof(bbs::BatchJob),
-                               hipMemcpyHostToDevice, s->stream[0]));
-    SHIP_TRY(s, hipMemcpyAsync(s->d_items.p, items.data(), items.size() * 4u,
-                               hipMemcpyHostToDevice, s->stream[0]));
-    bbs::BatchArgs a{};
-    a.jobs = (const bbs::BatchJob *)s->d_jobs.p;
-    a.items = (const uint32_t *)s->d_items.p;
-    a.n_items = (uint32_t)items.size();
-    a.cells = (const uint64_t *)s->cells.p;
-    a.am = (uint64_t *)s->am.p;
-    a.dm = (uint64_t *)s->dm.p;
-    a.best_am = (uint64_t *)s->best_am.p;
-    a.best_dm = (uint64_t *)s->best_dm.p;
-    a.beta = (float *)s->beta.p;
-    a.score = (float *)s->score.p;
-    a.best_beta = (float *)s->best_beta.p;
-    a.best_score = (float *)s->best_score.p;
-    a.best_step = (uint32_t *)s->best_step.p;
-    /* layouts spread over the streams after the uploads on stream 0 */
-    const uint32_t ns = s->cfg.n_streams;
-    SHIP_TRY(s, hipEventRecord(s->done_ev[0], s->stream[0]));
-    for (uint32_t i = 1; i < ns; i++) SHIP_TRY(s, hipStreamWaitEvent(s->stream[i], s->done_ev[0], 0));
-    const uint32_t wpg = s->cfg.waves_per_group;
-    while (s->range_ev.size() < 2u * ranges.size()) {
-      hipEvent_t e = nullptr;
-      SHIP_TRY(s, hipEventCreate(&e));
-      s->range_ev.push_back(e);
-    }
-    for (size_t r = 0; r < ranges.size(); r++) {
-      const R &rg = ranges[r];
-      uint32_t f = 0u, wpg_r = 1u;
-      range_launch(rg.shape, rg.max_nodes, wpg, false, UINT32_MAX, s->n_cus, f, wpg_r);
-      SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r], s->stream[r % ns]));
-      SHIP_TRY(s, bbs::launch_batch_items(a, rg.shape, rg.first, rg.count, wpg_r, f,
-                                          s->stream[r % ns]));
-      SHIP_TRY(s, hipEventRecord(s->range_ev[2u * r + 1u], s->stream[r % ns]));
-    }
-    for (uint32_t i = 1; i < ns; i++) {
-      SHIP_TRY(s, hipEventRecord(s->done_ev[i], s->stream[i]));
-      SHIP_TRY(s, hipStreamWaitEvent(s->stream[0], s->done_ev[i], 0));
-    }
-    SHIP_TRY(s, hipEventRecord(s->ev1, s->stream[0]));
-    SHIP_TRY(s, hipEventSynchronize(s->ev1));
-    const double t_end = srv_now(s);
-    s->busy_s += t_end - t_round;
-    s->rounds++;
-    /* the measured time of each layout's launch, per step and cost unit of
-     * its heaviest job, sizes that layout's next slices */
-    for (size_t r = 0; r < ranges.size(); r++) {
-      float ms = 0.f;
-      SHIP_TRY(s, hipEventElapsedTime(&ms, s->range_ev[2u * r], s->range_ev[2u * r + 1u]));
-      double heaviest = 0.0;
-      for (uint32_t i = ranges[r].first; i < ranges[r].first + ranges[r].count;) {
-        const uint32_t jr = items[i] >> 16;
-        heaviest = std::max(heaviest, work(jr));
-        i += jobs[jr].n_chains;
-      }
-      if (heaviest > 0.0 && ms > 0.f) {
-        const double sample = ms / heaviest;
-        auto f = s->ms_per_cost.find(ranges[r].shape);
-        if (f == s->ms_per_cost.end())
-          s->ms_per_cost[ranges[r].shape] = sample;
-        else
-          f->second = 0.5 * f->second + 0.5 * sample;
-      }
-    }
-    /* advance and retire */
-    std::vector<SrvJob *> still;
-    for (size_t r = 0; r < s->resident.size(); r++) {
-      SrvJob *j = s->resident[r];
-      s->steps_total += (uint64_t)jobs[r].n_chains * (jobs[r].step_end - jobs[r].step_begin);
-      j->steps_done = jobs[r].step_end;
-      j->started = true;
-      if (j->steps_done >= j->n_steps) {
-        rc = srv_retire(s, j, t_end);
-        if (rc) return rc;
-      } else {
-        still.push_back(j);
-      }
-    }
-    s->resident.swap(still);
-  }
-  if (n_resident) *n_resident = (uint32_t)s->resident.size();
+  if (n_resident) *n_resident = s->n_resident;
   if (n_pending) *n_pending = (uint32_t)s->pending.size();
   if (n_done_ready) *n_done_ready = (uint32_t)s->done.size();
   return BBS_OK;

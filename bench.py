@@ -26,9 +26,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The served stream launches one kernel per register layout each round, on
-# up to 16 streams; HIP runs them at once only on as many hardware queues
-# as the process asks for (read when HIP initialises, 4 by default).
+# The server runs one lane (stream) per register layout, on up to 16
+# streams; HIP runs them at once only on as many hardware queues as the
+# process asks for (read when HIP initialises, 4 by default).
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 from bbscite_amd import _capi as B  # noqa: E402
@@ -288,7 +288,7 @@ def serving_bench(n_jobs):
         "offered_steps_per_s": offered / span,
         "arrival_span_s": span,
         "makespan_s": end,
-        "rounds": rounds,
+        "slices": rounds,
     }
 
 

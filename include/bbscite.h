@@ -262,26 +262,30 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
 /* ---- serving: continuous batching of tree-search jobs ------------------
  * No reference counterpart: the reference runs one dataset per invocation
  * (src/main.cpp:159-169). A server keeps every admitted job's chains and
- * best trees on the device and advances all of them together, round by
- * round; jobs that arrive meanwhile are admitted as device memory and
- * concurrent-wave capacity free up. Every random draw is keyed by
+ * best trees on the device and advances them in slices. The resident jobs
+ * of one register layout form a lane: one kernel launch per slice on the
+ * lane's own stream, relaunched as soon as that slice ends, so layouts never
+ * wait for each other. Jobs that arrive meanwhile are admitted as device
+ * memory and concurrent-wave capacity free up. Every random draw is keyed by
  * (seed, job_tag, chain, step), so a job's result is the same alone
  * (bbs_create + bbs_run with job_tag 0) and in any mix. */
 typedef struct bbs_server_config {
   uint64_t hbm_budget_bytes;    /* device memory for resident jobs' state  */
-  uint32_t round_cost;          /* first round of a layout: a job advances
+  uint32_t round_cost;          /* first slice of a layout: a job advances
                                    round_cost / its step cost steps (the
                                    cost model counts about one unit per
                                    VALU instruction of a full rescoring)   */
-  float round_ms;               /* later rounds: steps sized so that each
+  float round_ms;               /* later slices: steps sized so that each
                                    job's slice takes about round_ms, from
                                    the measured time of its layout's last
-                                   launch                                  */
-  uint32_t max_resident_chains; /* 0: as many chain waves as the device
-                                   runs at once (LDS and registers)        */
+                                   slice                                   */
+  uint32_t max_resident_chains; /* 0: 16 times the chain waves the device
+                                   runs at once (two per SIMD); waves
+                                   beyond those are queued by the hardware,
+                                   and the LDS budget of admission grows by
+                                   the same factor                         */
   uint32_t waves_per_group;     /* chain waves per workgroup, 1..4          */
-  uint32_t n_streams;           /* 1..BBS_MAX_STREAMS: each round launches
-                                   one kernel per register layout, spread
+  uint32_t n_streams;           /* 1..BBS_MAX_STREAMS: lanes are dealt out
                                    over these streams; they run at once
                                    only on as many hardware queues as HIP
                                    gives the process (GPU_MAX_HW_QUEUES)   */
@@ -301,16 +305,18 @@ void bbs_server_config_default(bbs_server_config *cfg);
 int bbs_server_create(const bbs_server_config *cfg, bbs_server **out);
 /* Copies the job (parameters and data); *id identifies its result. */
 int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id);
-/* One round: admit pending jobs in arrival order while they fit, advance
- * every resident job, retire finished ones. Blocks until the round's
- * kernels are done. Counts may be NULL. */
+/* Collects the lanes whose slice has ended (retiring finished jobs), admits
+ * pending jobs in arrival order while they fit, and launches the next slice
+ * of every idle lane. If none of that was possible while slices run, waits
+ * for the oldest one to end first. Counts may be NULL. */
 int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
                     uint32_t *n_done_ready);
 /* Moves up to max finished results (in completion order) to out. */
 int bbs_server_poll(bbs_server *s, bbs_job_result *out, uint32_t max, uint32_t *n_out);
 /* Seconds since the server was created (the clock of the results). */
 double bbs_server_now(const bbs_server *s);
-/* Chain steps run so far, and the summed duration of the rounds. */
+/* Chain steps run so far, the time during which any slice ran, and the
+ * number of slices launched. */
 int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
                      uint64_t *rounds);
 const char *bbs_server_last_error(const bbs_server *s);
