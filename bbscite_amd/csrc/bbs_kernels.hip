@@ -689,11 +689,18 @@ __global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
       }
       return pair_score(n01, n11, __uint_as_float(q0.w), __uint_as_float(q1.w), L);
     };
-    uint4 A = rows[0];
+    const uint4 A = rows[0];
 #pragma unroll
     for (int c = 0; c < CMAX; c++) best[c] = pair(cl + Q * c, A);
-    for (uint32_t v = 1; v < N; v++) {
-      A = rows[v];
+    /* four nodes per pass share each cell's LDS reads; past the last node
+     * the clamped index repeats node N - 1, which cannot change a maximum
+     * (only node 0's NaN is special, handled below) */
+    const uint32_t last = N - 1u;
+    for (uint32_t v = 1; v < N; v += 4) {
+      const uint4 A0 = rows[v];
+      const uint4 A1 = rows[v + 1u < last ? v + 1u : last];
+      const uint4 A2 = rows[v + 2u < last ? v + 2u : last];
+      const uint4 A3 = rows[v + 3u < last ? v + 3u : last];
       /* an opaque base keeps the cell reads inside the node loop: hoisting
        * all of them would need CMAX * 4 Q registers */
       uint32_t base = 0u;
@@ -701,7 +708,12 @@ __global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
       const uint4 *cb = reinterpret_cast<const uint4 *>(
           reinterpret_cast<const char *>(cl) + base);
 #pragma unroll
-      for (int c = 0; c < CMAX; c++) best[c] = vmax(best[c], pair(cb + Q * c, A));
+      for (int c = 0; c < CMAX; c++) {
+        const uint4 *cp = cb + Q * c;
+        const float m01 = vmax(pair(cp, A0), pair(cp, A1));
+        const float m23 = vmax(pair(cp, A2), pair(cp, A3));
+        best[c] = vmax(best[c], vmax(m01, m23));
+      }
     }
     /* A NaN at node 0 sticks in the reference's maximum; it needs an
      * infinite log-probability, i.e. beta (or alpha) exactly 0 or 1. */
