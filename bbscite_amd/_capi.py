@@ -645,7 +645,8 @@ def server_config(**kw) -> ServerConfig:
 
 class Server:
     """A continuous-batching server (bbs_server_*): submit jobs at any time,
-    step() runs one round, poll() collects finished jobs."""
+    step() collects ended slices, admits and relaunches (one lane per register
+    layout), poll() collects finished jobs."""
 
     def __init__(self, cfg: ServerConfig = None):
         self.h = C.c_void_p()

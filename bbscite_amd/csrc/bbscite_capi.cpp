@@ -1686,7 +1686,7 @@ struct bbs_server {
   uint64_t next_id = 1;
   uint64_t lds_cap = 0, lds_used = 0;
   uint32_t chain_cap = 0, chains_used = 0, n_resident = 0, n_busy = 0;
-  uint64_t steps_total = 0, rounds = 0;
+  uint64_t steps_total = 0, slices = 0;
   double busy_s = 0.0, busy_since = 0.0;
   std::chrono::steady_clock::time_point t0;
   std::string err;
@@ -1941,7 +1941,7 @@ int srv_lane_launch(bbs_server *s, Lane *l) {
   SHIP_TRY(s, hipEventRecord(l->ev_end, l->stream));
   l->busy = true;
   if (s->n_busy++ == 0u) s->busy_since = srv_now(s);
-  s->rounds++;
+  s->slices++;
   return BBS_OK;
 }
 
@@ -2198,11 +2198,11 @@ int bbs_server_poll(bbs_server *s, bbs_job_result *out, uint32_t max, uint32_t *
 double bbs_server_now(const bbs_server *s) { return s ? srv_now(s) : 0.0; }
 
 int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
-                     uint64_t *rounds) {
+                     uint64_t *slices) {
   if (!s) return sfail(nullptr, BBS_ERR_ARG, "server is NULL");
   if (chain_steps) *chain_steps = s->steps_total;
   if (busy_s) *busy_s = s->busy_s;
-  if (rounds) *rounds = s->rounds;
+  if (slices) *slices = s->slices;
   return BBS_OK;
 }
 

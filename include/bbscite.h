@@ -318,7 +318,7 @@ double bbs_server_now(const bbs_server *s);
 /* Chain steps run so far, the time during which any slice ran, and the
  * number of slices launched. */
 int bbs_server_stats(const bbs_server *s, uint64_t *chain_steps, double *busy_s,
-                     uint64_t *rounds);
+                     uint64_t *slices);
 const char *bbs_server_last_error(const bbs_server *s);
 void bbs_server_destroy(bbs_server *s);
 

@@ -1,7 +1,8 @@
 """The serving loop (continuous batching): jobs submitted at different
-times, admitted as capacity frees up and advanced round by round together
-with whatever else is resident, must give exactly the result of the same
-job run alone through the plain path (bbs_create + bbs_run)."""
+times, admitted as capacity frees up and advanced slice by slice (one lane
+per register layout) alongside whatever else is resident, must give exactly
+the result of the same job run alone through the plain path (bbs_create +
+bbs_run)."""
 import numpy as np
 import pytest
 
@@ -43,7 +44,7 @@ JOBS = [  # G, C, chains, steps: every register layout family, uneven lengths
 
 @pytest.mark.gpu
 def test_served_results_equal_alone():
-    """Small rounds (many windows per job) and a small chain budget (jobs
+    """Small slices (many windows per job) and a small chain budget (jobs
     queue and are admitted as others finish); some jobs arrive while others
     run. Each result equals the job's plain run bit for bit."""
     say("[serve] mixed jobs")
@@ -55,20 +56,20 @@ def test_served_results_equal_alone():
     with B.Server(cfg) as srv:
         for i in range(4):
             ids[srv.submit(*jobs[i])] = i
-        rounds = 0
+        calls = 0
         nxt = 4
         while len(results) < len(jobs):
-            if nxt < len(jobs) and rounds % 3 == 2:
+            if nxt < len(jobs) and calls % 3 == 2:
                 ids[srv.submit(*jobs[nxt])] = nxt
                 nxt += 1
             res, pend, ready = srv.step()
-            rounds += 1
+            calls += 1
             for r in srv.poll():
                 results[ids[r.id]] = r
-            assert rounds < 20000
-        chain_steps, busy, n_rounds = srv.stats()
+            assert calls < 20000
+        chain_steps, busy, n_slices = srv.stats()
     assert chain_steps == sum(p.n_chains * p.n_steps for p, *_ in jobs)
-    assert n_rounds > len(jobs)
+    assert n_slices > len(jobs)
     say("[serve] plain runs")
     for i, (p, mut, kno, G) in enumerate(jobs):
         r = results[i]
@@ -81,6 +82,39 @@ def test_served_results_equal_alone():
         assert _bits(r.best_beta) == _bits(beta), i
         assert np.array_equal(r.rows("am"), am) and np.array_equal(r.rows("dm"), dm), i
         assert r.best_step == st[r.best_chain]
+
+
+@pytest.mark.gpu
+def test_served_lanes_with_default_configuration():
+    """The default server (a resident set larger than the device runs at
+    once, 16 streams): several jobs per register layout share a lane, and
+    a lane with few chains may switch to the pair-score table between
+    slices. Every result equals the job's plain run bit for bit."""
+    say("[serve] lanes, default configuration")
+    specs = [(94, 95, 40, 600), (90, 90, 24, 900), (94, 95, 8, 1500), (30, 40, 64, 700),
+             (28, 33, 16, 1200), (127, 128, 12, 500), (120, 100, 6, 800), (60, 60, 32, 650)]
+    jobs = [_job(50 + i, *spec) for i, spec in enumerate(specs)]
+    cfg = B.server_config(n_streams=16, round_cost=400_000, round_ms=2.0)
+    results = {}
+    ids = {}
+    with B.Server(cfg) as srv:
+        for i, j in enumerate(jobs):
+            ids[srv.submit(*j)] = i
+        calls = 0
+        while len(results) < len(jobs):
+            srv.step()
+            calls += 1
+            for r in srv.poll():
+                results[ids[r.id]] = r
+            assert calls < 100000
+    for i, (p, mut, kno, G) in enumerate(jobs):
+        r = results[i]
+        with B.Context(p, mut, kno, G) as ctx:
+            ctx.run()
+            am, dm, beta, score = ctx.best()
+        assert _bits(r.best_score) == _bits(score), i
+        assert _bits(r.best_beta) == _bits(beta), i
+        assert np.array_equal(r.rows("dm"), dm), i
 
 
 @pytest.mark.gpu
