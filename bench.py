@@ -209,6 +209,10 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
         total = b.total_steps
     rate_b = total / (ms / 1e3)
     rate_p = plain_steps / t_plain
+    # the reference algorithm's int work of one step of each dataset,
+    # N C (4W + 2), over the batched run
+    ops = sum(p.n_chains * p.n_steps * (G + 1) * int(mut.shape[0]) * (4 * ((G + 31) // 32) + 2)
+              for p, mut, kno, G in jobs)
     return {
         "workload": f"256 datasets (genes U[16,94], cells U[16,95]), 64 chains each; "
                     f"batched {steps_batched} steps, plain {steps_plain} steps per chain",
@@ -217,6 +221,8 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
         "speedup": rate_b / rate_p,
         "batched_ms": ms,
         "best_ll_identical_to_plain": same,
+        "alg_int_tops": ops / (ms / 1e3) / 1e12,
+        "roofline_frac_nominal": ops / (ms / 1e3) / 1e12 / P_INT32_TOPS,
         "short_run": {"steps": steps_plain, "eager_ms": eager_short,
                       "graph_replay_ms": graph_short,
                       "graph_best_ll_identical_to_plain": same_graph},
