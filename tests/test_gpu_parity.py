@@ -418,6 +418,31 @@ def test_batched_items_equal_plain_runs(scoring):
             assert _bits(got[1][3]) == _bits(plain[1][3])
 
 
+def test_batched_largest_layouts_equal_plain_runs():
+    """The 128-cell layouts with two node slots (70 KB tables) and the
+    paired-slot layout with four gene words: few items, so the batched form
+    takes the tables; every job equals its plain run."""
+    say("[batched largest layouts]")
+    jobs = []
+    for i, (G, C, R) in enumerate(((127, 128, 3), (100, 110, 2), (120, 96, 3), (65, 97, 2))):
+        m, _, _ = B.generate_dataset(70 + i, G, C, 1e-3, 0.2, 0.1)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=R, n_steps=300, seed=80 + i, alpha_mean=1e-3,
+                             beta_mean=0.25, prob_beta_change=0.1)
+        jobs.append((p, mut, kno, G))
+    with B.Batch(jobs, B.batch_config(waves_per_group=2, n_streams=4)) as b:
+        b.run()
+        got = [(b.chain_results(j), b.best(j)) for j in range(len(jobs))]
+    for j, (p, mut, kno, G) in enumerate(jobs):
+        with B.Context(p, mut, kno, G) as ctx:
+            ctx.run()
+            plain = ctx.chain_results(), ctx.best()
+        for x, y in zip(got[j][0], plain[0]):
+            assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                                  _bits(y) if y.dtype == np.float32 else y), j
+        assert np.array_equal(got[j][1][1], plain[1][1]) and _bits(got[j][1][3]) == _bits(plain[1][3])
+
+
 def test_batched_items_configs_agree_and_autotune():
     say("[batched configs]")
     jobs = _mix(6, 77, 60)
