@@ -33,6 +33,7 @@ MOVE_SWAP_SUBTREES = 3
 
 SCORE_NO_PRIOR = 1
 SCORE_CELL_LANES = 2
+SCORE_VALU_LANES = 4
 
 PURPOSE_INIT_TREE = 2
 PURPOSE_TREE_SAMPLE = 6
@@ -481,21 +482,23 @@ class Context:
         return out
 
     def score_trees(self, dm_rows, betas=None, with_prior=True, counts=False,
-                    cell_lanes=False):
+                    cell_lanes=False, valu_lanes=False):
         dm_rows = np.ascontiguousarray(dm_rows, np.uint64)
         T = dm_rows.shape[0]
         ll = np.zeros(T, np.float32)
         b = None if betas is None else np.ascontiguousarray(betas, np.float32)
         cnt = np.zeros((T, self.n_cells, 4), np.uint8) if counts else None
-        flags = (0 if with_prior else SCORE_NO_PRIOR) | (SCORE_CELL_LANES if cell_lanes else 0)
+        flags = ((0 if with_prior else SCORE_NO_PRIOR) | (SCORE_CELL_LANES if cell_lanes else 0)
+                 | (SCORE_VALU_LANES if valu_lanes else 0))
         self._c(lib().bbs_score_trees(self.h, _ptr(dm_rows), T, _ptr(b), flags, _ptr(ll), _ptr(cnt)))
         return (ll, cnt) if counts else ll
 
     def bench_score_random(self, seed, n_trees, reps, with_prior=False, n_head=0,
-                           cell_lanes=False):
+                           cell_lanes=False, valu_lanes=False):
         ms = np.zeros(reps, np.float32)
         head = np.zeros(n_head, np.float32)
-        flags = (0 if with_prior else SCORE_NO_PRIOR) | (SCORE_CELL_LANES if cell_lanes else 0)
+        flags = ((0 if with_prior else SCORE_NO_PRIOR) | (SCORE_CELL_LANES if cell_lanes else 0)
+                 | (SCORE_VALU_LANES if valu_lanes else 0))
         self._c(
             lib().bbs_bench_score_random(
                 self.h, seed, n_trees, reps, flags, _ptr(ms), _ptr(head) if n_head else None, n_head

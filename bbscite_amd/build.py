@@ -27,6 +27,9 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs: the bulk scorer reads every count on the
+    # vector unit, so accumulators in AGPRs would cost a copy each
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
 ]
 LIB_SOURCES = ["bbs_kernels.hip", "bbscite_capi.cpp"]

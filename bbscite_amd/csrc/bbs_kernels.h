@@ -102,7 +102,7 @@ uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells);
 uint32_t chain_static_lds();
 /* cells32: [128][10] words for the lane-per-tree kernel (used when counts
  * are not requested), or null to use the lane-per-cell kernel. */
-hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
+hipError_t launch_score(const ScoreArgs &a, bool counts, bool mfma, hipStream_t stream,
                         const uint32_t *cells32);
 hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
 /* p[0..n) = value with a kernel (capturable in a hipGraph, unlike a memset) */

@@ -735,6 +735,130 @@ __global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
   }
 }
 
+/* ------------------------------------------ bulk score, matrix cores */
+
+typedef int bbs_v4i __attribute__((ext_vector_type(4)));
+typedef int bbs_v16i __attribute__((ext_vector_type(16)));
+
+/* Bits 0..15 of x as 16 bytes 0/1 (byte b = bit b): each nibble times
+ * 0x00204081 places its bits 0..3 at bits 0, 8, 16, 24 without carries. */
+__device__ __forceinline__ bbs_v4i bits_to_bytes16(uint32_t x) {
+  bbs_v4i r;
+#pragma unroll
+  for (int q = 0; q < 4; q++) r[q] = (int)((((x >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u);
+  return r;
+}
+
+/*
+ * TreeScorer::logscore_tree over many trees, one tree per wavefront, with
+ * the pair counts on the matrix cores: n11 = (K&M) A^T and n01 = (K&~M) A^T
+ * are products of 0/1 matrices (genes as the inner dimension), computed by
+ * v_mfma_i32_32x32x32_i8 on 32 x 32 tiles (rows = nodes, columns = cells).
+ * The float part of each pair stays on the vector unit in the reference's
+ * order (pair_score). Operands, per lane l (h = l / 32):
+ *   A (tree, per node tile): node 32 mt + l % 32, genes 32 kb + 16 h + b
+ *   B (data, per cell tile): cell 32 jt + l % 32, the same genes
+ *   D: cell 32 jt + l % 32, node 32 mt + 8 (r / 4) + 4 h + r % 4 in reg r.
+ * A and B put gene g at the same (half, byte), which is all a product needs.
+ * Each lane thus holds one cell's pairs for 16 nodes per tile: the maximum
+ * over nodes stays in the lane, the other 16 nodes are in lane l ^ 32.
+ * Padding nodes (>= n_nodes) are left out of the maximum; padding cells have
+ * no known entries and add +0.0. A NaN at node 0 sticks, as in the
+ * reference's first-wins maximum; later NaNs are ignored.
+ */
+template <int KB, int JT>
+__global__ __launch_bounds__(256) void score_mfma_kernel(ScoreArgs a,
+                                                         const uint32_t *__restrict__ cells32) {
+  __shared__ __align__(16) float sums[4][128];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, h = lane >> 5;
+  const uint32_t N = a.n_nodes;
+  float *const S = sums[wave];
+  /* the data operands and per-cell popcounts, once per wave */
+  bbs_v4i bkm[JT][KB], bknm[JT][KB];
+  float fa[JT], fb[JT];
+#pragma unroll
+  for (int jt = 0; jt < JT; jt++) {
+    const uint32_t *g = cells32 + 10u * (32u * (uint32_t)jt + (lane & 31u));
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+      bkm[jt][kb] = bits_to_bytes16((g[kb] >> (16u * h)) & 0xFFFFu);
+      bknm[jt][kb] = bits_to_bytes16((g[4 + kb] >> (16u * h)) & 0xFFFFu);
+    }
+    fa[jt] = __uint_as_float(g[8]);
+    fb[jt] = __uint_as_float(g[9]);
+  }
+  const uint32_t waves_total = gridDim.x * 4u;
+  for (uint32_t tree = blockIdx.x * 4u + wave; tree < a.n_trees; tree += waves_total) {
+    LogProbs L;
+    L.L00 = a.L00;
+    L.L10 = a.L10;
+    float prior;
+    if (a.betas) {
+      const float beta = a.betas[tree];
+      L.L01 = bbs_logf(beta);
+      L.L11 = bbs_logf_of_double(1.0 - (double)beta);
+      prior = logscore_beta(a.prior, beta);
+    } else {
+      L.L01 = a.L01;
+      L.L11 = a.L11;
+      prior = a.prior_value;
+    }
+    /* the tree operand: node rows as bytes, node tiles x gene blocks */
+    bbs_v4i at[KB][KB];
+#pragma unroll
+    for (int mt = 0; mt < KB; mt++) {
+      uint32_t n = 32u * (uint32_t)mt + (lane & 31u);
+      n = n < N ? n : N - 1u;
+      const uint4 row = reinterpret_cast<const uint4 *>(a.dm)[(uint64_t)tree * N + n];
+      const uint32_t w[4] = {row.x, row.y, row.z, row.w};
+#pragma unroll
+      for (int kb = 0; kb < KB; kb++) at[mt][kb] = bits_to_bytes16((w[kb] >> (16u * h)) & 0xFFFFu);
+    }
+    const float ninf = __int_as_float(0xff800000);
+#pragma unroll
+    for (int jt = 0; jt < JT; jt++) {
+      float best = ninf;
+      bool nan0 = false;
+#pragma unroll
+      for (int mt = 0; mt < KB; mt++) {
+        bbs_v16i c11 = {}, c01 = {};
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+          c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(at[mt][kb], bkm[jt][kb], c11, 0, 0, 0);
+          c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(at[mt][kb], bknm[jt][kb], c01, 0, 0, 0);
+        }
+        /* padding nodes exist only in a tile reaching past n_nodes */
+        auto tile_pairs = [&](bool full) {
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const uint32_t node = 32u * (uint32_t)mt + 8u * (uint32_t)(r / 4) + 4u * h +
+                                  (uint32_t)(r % 4);
+            const float sc = pair_score((uint32_t)c01[r], (uint32_t)c11[r], fa[jt], fb[jt], L);
+            if (mt == 0 && r == 0) nan0 = h == 0u && sc != sc;
+            best = vmax(best, (full || node < N) ? sc : ninf);
+          }
+        };
+        if (32u * (uint32_t)(mt + 1) <= N)
+          tile_pairs(true);
+        else
+          tile_pairs(false);
+      }
+      /* the cell's other 16 nodes per tile are in lane l ^ 32 */
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(best),
+                                                       __float_as_uint(best), false, false);
+      const float other = __uint_as_float(sw[1]);
+      const float qnan = __int_as_float(0x7fc00000);
+      const uint32_t cell = 32u * (uint32_t)jt + lane;
+      if (lane < 32u) S[cell] = cell < a.n_cells ? (nan0 ? qnan : vmax(best, other)) : 0.0f;
+    }
+    wave_lds_sync();
+    /* ((0 + x[0]) + x[1]) + ... in cell order */
+    const float T = sum_padded<JT * 8>(S);
+    wave_lds_sync();
+    if (lane == 0u) a.ll[tree] = a.with_prior ? T + prior : T;
+  }
+}
+
 /* ------------------------------------------------------- random trees */
 
 /* One uniform random tree from a Pruefer code drawn from the Philox stream
@@ -961,11 +1085,27 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_score(const ScoreArgs &a, bool counts, hipStream_t stream,
+hipError_t launch_score(const ScoreArgs &a, bool counts, bool mfma, hipStream_t stream,
                         const uint32_t *cells32) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, false);
   if (!counts && cells32 != nullptr) {
+    /* matrix-core counts: one tree per wave, gene blocks x cell tiles */
+    if (mfma) {
+      const int kb = (int)((a.n_nodes + 31u) / 32u), jt = (int)((a.n_cells + 31u) / 32u);
+      uint32_t groups = (a.n_trees + 3u) / 4u;
+      if (groups > 16384u) groups = 16384u;
+      if (groups == 0u) return hipSuccess;
+#define XM(KB_, JT_)                                                           \
+  if (kb == KB_ && jt == JT_) {                                                \
+    hipLaunchKernelGGL((score_mfma_kernel<KB_, JT_>), dim3(groups), dim3(256), \
+                       0, stream, a, cells32);                                 \
+    return hipGetLastError();                                                  \
+  }
+      XM(1, 1) XM(1, 2) XM(1, 3) XM(1, 4) XM(2, 1) XM(2, 2) XM(2, 3) XM(2, 4)
+      XM(3, 1) XM(3, 2) XM(3, 3) XM(3, 4) XM(4, 1) XM(4, 2) XM(4, 3) XM(4, 4)
+#undef XM
+    }
     const int cmax = a.n_cells <= 32u ? 32 : a.n_cells <= 64u ? 64 : a.n_cells <= 96u ? 96 : 128;
     const int wq = (int)((a.n_nodes - 1u + 31u) / 32u);
     uint32_t groups = (a.n_trees + 255u) / 256u;

@@ -696,7 +696,7 @@ int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
   if (e == hipSuccess && betas)
     e = hipMemcpy(d_betas.p, betas, (size_t)n_trees * 4u, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = bbs::launch_score(s, counts != nullptr, ctx->stream,
+    e = bbs::launch_score(s, counts != nullptr, !(flags & BBS_SCORE_VALU_LANES), ctx->stream,
                           (flags & BBS_SCORE_CELL_LANES) ? nullptr
                                                          : (const uint32_t *)ctx->cells32.p);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -748,7 +748,7 @@ int bbs_bench_score_random(bbs_ctx *ctx, uint64_t seed, uint32_t n_trees,
   for (uint32_t r = 0; r < reps && e == hipSuccess; r++) {
     e = hipEventRecord(ctx->ev0, ctx->stream);
     if (e == hipSuccess)
-      e = bbs::launch_score(s, false, ctx->stream,
+      e = bbs::launch_score(s, false, !(flags & BBS_SCORE_VALU_LANES), ctx->stream,
                             (flags & BBS_SCORE_CELL_LANES) ? nullptr
                                                            : (const uint32_t *)ctx->cells32.p);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);

@@ -41,6 +41,8 @@ W32 = (G + 31) // 32
 ALG_INT_OPS = N * C * (4 * W32 + 2)  # 126,350 at 94 x 95
 # gfx950 int32 VALU peak: 256 CUs x 64 lanes x 2.4 GHz.
 P_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
+# MI355X dense int8 MFMA peak (published), in tera-ops (2 per multiply-add)
+P_INT8_MFMA_TOPS = 5000.0
 # HBM bytes of one 100,000-step chain launch at 94 x 95 with 1,024 chains,
 # from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors a
 # plain 1 GiB device copy gives those counters: 1,975 KiB x 2 + 6,304 KiB.
@@ -120,12 +122,22 @@ def scoring_bench(reps=5, peak=P_INT32_TOPS):
         ms, _ = ctx.bench_score_random(3, n_trees, reps, with_prior=False, n_head=0)
     k_ms = float(np.median(ms))
     trees_s = n_trees / (k_ms / 1e3)
-    achieved = ALG_INT_OPS * trees_s / 1e12
+    # The pair counts run on the matrix cores (int8 MFMA over 96 x 96 x 96
+    # tiles, two matrices); what bounds the kernel is the float part of each
+    # (cell, node) pair on the vector unit: 2 cvt, 2 sub, 4 mul, 3 add, 1 max.
+    fp_ops = N * C * 12
+    achieved = fp_ops * trees_s / 1e12
+    macs = 2 * 96 * 96 * 96
     return {
         "trees_per_s": trees_s,
         "kernel_ms": k_ms,
-        "roofline": {"bound": "valu-int32", "achieved": achieved, "peak": peak,
-                     "unit": "TOP/s", "frac": achieved / peak},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": peak,
+                     "unit": "TOP/s", "frac": achieved / peak,
+                     "alg_valu_ops_per_tree": fp_ops,
+                     "mfma_int8": {"achieved_tops": 2 * macs * trees_s / 1e12,
+                                   "peak_tops": P_INT8_MFMA_TOPS,
+                                   "frac": 2 * macs * trees_s / 1e12 / P_INT8_MFMA_TOPS}},
+        "alg_int_tops": ALG_INT_OPS * trees_s / 1e12,
         "counted_bits_per_s": trees_s * 4 * N ** 3,
     }
 

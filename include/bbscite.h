@@ -153,9 +153,12 @@ int bbs_get_chain_states(bbs_ctx *ctx, uint64_t *am, uint64_t *dm);
 int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out);
 
 #define BBS_SCORE_NO_PRIOR 1u
-/* Use the lane-per-cell scorer (the chain kernel's) instead of the
- * lane-per-tree bulk scorer; results are identical. */
+/* Use the lane-per-cell scorer (the chain kernel's) instead of the bulk
+ * scorer; results are identical. */
 #define BBS_SCORE_CELL_LANES 2u
+/* Bulk scorer with the pair counts on the vector unit (one tree per lane)
+ * instead of the matrix cores (one tree per wave); results are identical. */
+#define BBS_SCORE_VALU_LANES 4u
 
 /*
  * Replaces TreeScorer::logscore_tree (src/TreeScorer.hpp:131-210) over many

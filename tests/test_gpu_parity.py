@@ -59,11 +59,15 @@ def test_score_trees_bit_exact(G, C):
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     with B.Context(p, mut, kno, G) as ctx:
         ll, cnt = ctx.score_trees(dm, with_prior=False, counts=True)  # lane per cell
-        ll_lanes = ctx.score_trees(dm, with_prior=False)  # lane per tree
+        ll_mfma = ctx.score_trees(dm, with_prior=False)  # matrix-core counts, wave per tree
+        ll_lanes = ctx.score_trees(dm, with_prior=False, valu_lanes=True)  # lane per tree
         llb = ctx.score_trees(dm, betas=betas, with_prior=True)
         llb_cells = ctx.score_trees(dm, betas=betas, with_prior=True, cell_lanes=True)
+        llb_lanes = ctx.score_trees(dm, betas=betas, with_prior=True, valu_lanes=True)
+    assert np.array_equal(_bits(ll), _bits(ll_mfma))
     assert np.array_equal(_bits(ll), _bits(ll_lanes))
     assert np.array_equal(_bits(llb), _bits(llb_cells))
+    assert np.array_equal(_bits(llb), _bits(llb_lanes))
     for t in range(n_trees):
         tree = O.tree_from_rows(np.zeros_like(dm[t]), dm[t], p.beta_mean)  # scorer reads dm
         T, ocnt, _ = O.logscore_tree(s, tree, with_prior=False, counts=True)
@@ -292,7 +296,7 @@ def test_caller_initial_states():
 
 def test_score_trees_degenerate_beta():
     """beta exactly 0 or 1 gives infinite log-probabilities; NaN pairs at
-    node 0 stick in the maximum as in the reference (both scorers)."""
+    node 0 stick in the maximum as in the reference (all three scorers)."""
     G, C = 10, 12
     m = _iid_matrix(5, G, C)
     mut, kno = B.encode_matrix(m)
@@ -303,10 +307,11 @@ def test_score_trees_degenerate_beta():
     with B.Context(p, mut, kno, G) as ctx:
         a = ctx.score_trees(dm, betas=betas, with_prior=False)
         b = ctx.score_trees(dm, betas=betas, with_prior=False, cell_lanes=True)
+        c = ctx.score_trees(dm, betas=betas, with_prior=False, valu_lanes=True)
     for t in range(70):
         tree = O.tree_from_rows(np.zeros_like(dm[t]), dm[t], float(betas[t]))
         ref = O.logscore_tree(s, tree, with_prior=False)
-        for got in (a[t], b[t]):
+        for got in (a[t], b[t], c[t]):
             if math.isnan(ref):
                 assert math.isnan(got), t
             else:
@@ -323,8 +328,11 @@ def test_bench_score_random_head_matches_oracle():
         ms, head = ctx.bench_score_random(3, 4096, 2, with_prior=False, n_head=32)
         ms2, head2 = ctx.bench_score_random(3, 4096, 1, with_prior=False, n_head=32,
                                             cell_lanes=True)
+        ms3, head3 = ctx.bench_score_random(3, 4096, 1, with_prior=False, n_head=32,
+                                            valu_lanes=True)
     assert (ms > 0).all()
     assert np.array_equal(_bits(head), _bits(head2))
+    assert np.array_equal(_bits(head), _bits(head3))
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     dm = B.random_trees(3, G, 32)
     rate, ll = O.bench_score(s, dm, p.beta_mean, 1)
