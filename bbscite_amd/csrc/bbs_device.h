@@ -365,6 +365,22 @@ __device__ __forceinline__ void store_pen(const bool (&diff)[NS], uint32_t n_nod
   }
 }
 
+/* The ordered cell sum of S.sums (src/TreeScorer.hpp:198-205): the layout's
+ * cells, +0.0 past n_cells. */
+template <int CS>
+__device__ __forceinline__ float ordered_sum(uint32_t n_cells, WaveLds &S) {
+  wave_lds_sync();
+  float T;
+  if constexpr (CS == 1) {
+    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
+                                                                 : sum_padded<16>(S.sums);
+  } else {
+    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
+  }
+  wave_lds_sync();
+  return rfl_f(T);
+}
+
 /* Per-cell maxima -> ordered cell sum (src/TreeScorer.hpp:198-205).
  * best[j]/nan0[j] of the full slots, and for the paired slot the maxima of
  * the two lane halves. Cells past n_cells store +0.0: the running sum starts
@@ -387,16 +403,7 @@ __device__ __forceinline__ float finish_sum(const float (&best)[2], const bool (
   } else if (SL > 1) {
     S.sums[64u + lane] = 64u + lane < n_cells ? (nan0[1] ? qnan : best[1]) : 0.0f;
   }
-  wave_lds_sync();
-  float T;
-  if constexpr (CS == 1) {
-    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
-                                                                 : sum_padded<16>(S.sums);
-  } else {
-    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
-  }
-  wave_lds_sync();
-  return rfl_f(T);
+  return ordered_sum<CS>(n_cells, S);
 }
 
 /* Node loop of the full scorer over the rows in S.rows: per cell the
@@ -492,6 +499,139 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
       full(B2, clampn(v + 1u), false);
     }
   }
+}
+
+/* ---- pair counts on the matrix cores (v_mfma_i32_32x32x32_i8) ---------
+ * n11 = (K&M) A^T and n01 = (K&~M) A^T over 32 x 32 tiles, rows = nodes,
+ * columns = cells, genes as the inner dimension. Lane l (h = l / 32) holds
+ * in A node 32 mt + l % 32 and in B cell 32 jt + l % 32, each with genes
+ * 32 kb + 16 h + b in byte b; in D cell 32 jt + l % 32 and node
+ * 32 mt + 8 (r / 4) + 4 h + r % 4 in register r. The root's bit (node
+ * n_nodes - 1) meets a zero in K&M and K&~M, so W gene words cover the
+ * inner dimension. */
+typedef int bbs_v4i __attribute__((ext_vector_type(4)));
+typedef int bbs_v16i __attribute__((ext_vector_type(16)));
+
+/* Bits 0..15 of x as 16 bytes 0/1 (byte b = bit b): each nibble times
+ * 0x00204081 places its bits 0..3 at bits 0, 8, 16, 24 without carries. */
+__device__ __forceinline__ bbs_v4i bits_to_bytes16(uint32_t x) {
+  bbs_v4i r;
+#pragma unroll
+  for (int q = 0; q < 4; q++) r[q] = (int)((((x >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u);
+  return r;
+}
+
+/* The data operands of a dataset (cells of the layout in tiles of 32). */
+template <int CS, int W>
+struct MfmaCells {
+  static constexpr int JT = CS == 1 ? 2 : CS == 3 ? 3 : 4;
+  bbs_v4i km[JT][W], knm[JT][W];
+  float fa[JT], fb[JT]; /* |K&~M|, |K&M| of the lane's column cell */
+};
+
+/* cells: [n_cells][4] words (K&M lo, K&M hi, K&~M lo, K&~M hi). */
+template <int CS, int W>
+__device__ __forceinline__ void load_mfma_cells(const uint64_t *cells, uint32_t n_cells,
+                                                MfmaCells<CS, W> &m) {
+  const uint32_t lane = lane_id(), h = lane >> 5;
+#pragma unroll
+  for (int jt = 0; jt < MfmaCells<CS, W>::JT; jt++) {
+    const uint32_t c = 32u * (uint32_t)jt + (lane & 31u);
+    uint64_t q[4] = {0ull, 0ull, 0ull, 0ull};
+    if (c < n_cells) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) q[i] = cells[4u * c + (uint32_t)i];
+    }
+    const uint32_t km32[4] = {(uint32_t)q[0], (uint32_t)(q[0] >> 32), (uint32_t)q[1],
+                              (uint32_t)(q[1] >> 32)};
+    const uint32_t knm32[4] = {(uint32_t)q[2], (uint32_t)(q[2] >> 32), (uint32_t)q[3],
+                               (uint32_t)(q[3] >> 32)};
+    uint32_t b = 0u, a = 0u;
+#pragma unroll
+    for (int kb = 0; kb < W; kb++) {
+      m.km[jt][kb] = bits_to_bytes16((km32[kb] >> (16u * h)) & 0xFFFFu);
+      m.knm[jt][kb] = bits_to_bytes16((knm32[kb] >> (16u * h)) & 0xFFFFu);
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      b += (uint32_t)__popc(km32[w]);
+      a += (uint32_t)__popc(knm32[w]);
+    }
+    m.fa[jt] = (float)a;
+    m.fb[jt] = (float)b;
+  }
+}
+
+/* Every pair score of the tree in S.rows into the table, the counts on the
+ * matrix cores and the float part as pair_score. With SUM, the per-cell
+ * maxima (node 0 seeds them: its NaN sticks, later NaNs are ignored; the
+ * maximum is order-free otherwise) go to S.sums and the ordered cell sum
+ * is returned, as score_tree with a table. */
+template <int NS, int CS, int W, bool SUM>
+__device__ __forceinline__ float mfma_maxima(const MfmaCells<CS, W> &mc, uint32_t n_nodes,
+                                             uint32_t n_cells, const LogProbs &L, WaveLds &S,
+                                             TableView tab) {
+  constexpr int MT = 2 * NS, JT = MfmaCells<CS, W>::JT;
+  const uint32_t lane = lane_id(), h = lane >> 5;
+  const uint32_t last = n_nodes - 1u;
+  bbs_v4i at[MT][W];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) {
+    uint32_t n = 32u * (uint32_t)mt + (lane & 31u);
+    n = n < last ? n : last;
+    const uint4 row = S.rows[n];
+    const uint32_t w[4] = {row.x, row.y, row.z, row.w};
+#pragma unroll
+    for (int kb = 0; kb < W; kb++) at[mt][kb] = bits_to_bytes16((w[kb] >> (16u * h)) & 0xFFFFu);
+  }
+  const float ninf = __int_as_float(0xff800000);
+  const float qnan = __int_as_float(0x7fc00000);
+#pragma unroll
+  for (int jt = 0; jt < JT; jt++) {
+    const uint32_t cell = 32u * (uint32_t)jt + (lane & 31u);
+    if (32u * (uint32_t)jt >= n_cells) {
+      if (SUM && lane < 32u) S.sums[cell] = 0.0f;
+      continue;
+    }
+    float *const trow = tab.t + cell * tab.stride;
+    float best = ninf;
+    bool nan0 = false;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+      if (32u * (uint32_t)mt >= n_nodes) break;
+      bbs_v16i c11 = {}, c01 = {};
+#pragma unroll
+      for (int kb = 0; kb < W; kb++) {
+        c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(at[mt][kb], mc.km[jt][kb], c11, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(at[mt][kb], mc.knm[jt][kb], c01, 0, 0, 0);
+      }
+      auto tile_pairs = [&](bool full) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const uint32_t node = 32u * (uint32_t)mt + 8u * (uint32_t)(r / 4) + 4u * h +
+                                (uint32_t)(r % 4);
+          const float sc = pair_score((uint32_t)c01[r], (uint32_t)c11[r], mc.fa[jt], mc.fb[jt], L);
+          if (mt == 0 && r == 0) nan0 = h == 0u && sc != sc;
+          const bool valid = full || node < n_nodes;
+          if (valid) trow[node] = sc;
+          if (SUM) best = vmax(best, valid ? sc : ninf);
+        }
+      };
+      if (32u * (uint32_t)(mt + 1) <= n_nodes)
+        tile_pairs(true);
+      else
+        tile_pairs(false);
+    }
+    if (SUM) {
+      /* the cell's other nodes of each tile are in lane l ^ 32 */
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(best),
+                                                       __float_as_uint(best), false, false);
+      const float other = __uint_as_float(sw[1]);
+      if (lane < 32u) S.sums[cell] = cell < n_cells ? (nan0 ? qnan : vmax(best, other)) : 0.0f;
+    }
+  }
+  if (!SUM) return 0.0f;
+  return ordered_sum<CS>(n_cells, S);
 }
 
 /*

@@ -178,6 +178,13 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
 
   Cells<CS, W> cells;
   load_cells<CS, W>(a.cells, a.n_cells, cells);
+  /* With the table, full scorings (the first one and beta moves) count
+   * pairs on the matrix cores when the dataset fills its 32 x 32 tiles
+   * well; padded pairs cost the vector unit as much as real ones. */
+  const uint32_t tiles_area = 1024u * ((N + 31u) >> 5) * ((a.n_cells + 31u) >> 5);
+  const bool use_mfma = INCR && 10u * N * a.n_cells >= 7u * tiles_area;
+  MfmaCells<CS, W> mc;
+  if (use_mfma) load_mfma_cells<CS, W>(a.cells, a.n_cells, mc);
 
   uint64_t am[NS][NS], dm[NS][NS], bam[NS][NS], bdm[NS][NS];
 #pragma unroll
@@ -200,8 +207,14 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   uint32_t unused01[2], unused11[2];
   float cur_score;
   if (a.score_init || INCR) {
-    const float t = score_tree<NS, CS, W, false, INCR>(dm, cells, N, a.n_cells, L, lds, unused01,
-                                                 unused11, INCR ? tab : TableView{nullptr, 0u});
+    float t;
+    if (use_mfma) {
+      store_rows<NS>(dm, N, lds);
+      t = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, L, lds, tab);
+    } else {
+      t = score_tree<NS, CS, W, false, INCR>(dm, cells, N, a.n_cells, L, lds, unused01,
+                                             unused11, INCR ? tab : TableView{nullptr, 0u});
+    }
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
     cur_score = *a.score;
@@ -343,8 +356,14 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       /* beta moves (or full scoring): with INCR the proposal's pair scores
        * go straight into the table, since most beta moves are accepted; a
        * rejection rebuilds the table from the current beta */
-      prop_raw = score_tree<NS, CS, W, false, INCR>(pdm, cells, N, a.n_cells, Lp, lds, unused01,
-                                              unused11, INCR ? tab : TableView{nullptr, 0u});
+      if (use_mfma) {
+        store_rows<NS>(pdm, N, lds);
+        prop_raw = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, Lp, lds, tab);
+      } else {
+        prop_raw = score_tree<NS, CS, W, false, INCR>(pdm, cells, N, a.n_cells, Lp, lds,
+                                                      unused01, unused11,
+                                                      INCR ? tab : TableView{nullptr, 0u});
+      }
     }
     const float prop_score = prop_raw + prop_prior;
 
@@ -385,9 +404,13 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     if (INCR && m.type == BBS_MOVE_CHANGE_BETA && !accept) {
       /* the table holds the rejected beta's scores: back to the current
        * beta (the rows in lds.rows are the current tree's) */
-      float best[2];
-      bool n0[2];
-      node_maxima<NS, CS, W, false, true>(cells, N, L, lds, tab, best, n0, unused01, unused11);
+      if (use_mfma) {
+        mfma_maxima<NS, CS, W, false>(mc, N, a.n_cells, L, lds, tab);
+      } else {
+        float best[2];
+        bool n0[2];
+        node_maxima<NS, CS, W, false, true>(cells, N, L, lds, tab, best, n0, unused01, unused11);
+      }
       wave_lds_sync();
     }
     if (accept) {
@@ -736,18 +759,6 @@ __global__ __launch_bounds__(256) void score_lanes_kernel(ScoreArgs a,
 }
 
 /* ------------------------------------------ bulk score, matrix cores */
-
-typedef int bbs_v4i __attribute__((ext_vector_type(4)));
-typedef int bbs_v16i __attribute__((ext_vector_type(16)));
-
-/* Bits 0..15 of x as 16 bytes 0/1 (byte b = bit b): each nibble times
- * 0x00204081 places its bits 0..3 at bits 0, 8, 16, 24 without carries. */
-__device__ __forceinline__ bbs_v4i bits_to_bytes16(uint32_t x) {
-  bbs_v4i r;
-#pragma unroll
-  for (int q = 0; q < 4; q++) r[q] = (int)((((x >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u);
-  return r;
-}
 
 /*
  * TreeScorer::logscore_tree over many trees, one tree per wavefront, with
