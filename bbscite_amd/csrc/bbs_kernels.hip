@@ -170,8 +170,11 @@ struct ChainView {
  * the pair scores of the current tree are kept in the LDS table `tab`: a
  * tree move rescores only the nodes whose ancestor row changed and takes
  * the other nodes' scores from the table, which is updated when the move is
- * accepted; beta moves rescore every pair. */
-template <int NS, int CS, int W, bool TRACE, bool INCR>
+ * accepted; beta moves rescore every pair. With MFMA those full scorings
+ * count pairs on the matrix cores where the dataset fills its tiles (the
+ * plain kernel; the batched kernel, mostly small datasets, keeps its
+ * registers for the vector path). */
+template <int NS, int CS, int W, bool TRACE, bool INCR, bool MFMA>
 __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, TableView tab) {
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
@@ -182,7 +185,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
    * pairs on the matrix cores when the dataset fills its 32 x 32 tiles
    * well; padded pairs cost the vector unit as much as real ones. */
   const uint32_t tiles_area = 1024u * ((N + 31u) >> 5) * ((a.n_cells + 31u) >> 5);
-  const bool use_mfma = INCR && 10u * N * a.n_cells >= 7u * tiles_area;
+  const bool use_mfma = MFMA && INCR && 10u * N * a.n_cells >= 7u * tiles_area;
   MfmaCells<CS, W> mc;
   if (use_mfma) load_mfma_cells<CS, W>(a.cells, a.n_cells, mc);
 
@@ -518,12 +521,12 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   if constexpr (INCR) {
     const ChainLds l = ChainLds::of(a.n_nodes, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
-    run_chain<NS, CS, W, TRACE, true>(
+    run_chain<NS, CS, W, TRACE, true, true>(
         v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes)});
   } else {
     __shared__ WaveLdsStore store;
     WaveLds lds = store.view();
-    run_chain<NS, CS, W, TRACE, false>(v, lds, TableView{nullptr, 0u});
+    run_chain<NS, CS, W, TRACE, false, false>(v, lds, TableView{nullptr, 0u});
   }
 }
 
@@ -578,12 +581,12 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
     float *slab = dyn + wave * wave_floats;
     const ChainLds l = ChainLds::of(J.n_nodes, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
-    run_chain<NS, CS, W, false, true>(v, lds,
+    run_chain<NS, CS, W, false, true, false>(v, lds,
                                       TableView{slab + l.table, TableView::stride_for(J.n_nodes)});
   } else {
     __shared__ WaveLdsStore lds_all[4];
     WaveLds lds = lds_all[wave].view();
-    run_chain<NS, CS, W, false, false>(v, lds, TableView{nullptr, 0u});
+    run_chain<NS, CS, W, false, false, false>(v, lds, TableView{nullptr, 0u});
   }
 }
 
