@@ -45,8 +45,8 @@ P_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12
 P_INT8_MFMA_TOPS = 5000.0
 # HBM bytes of one 100,000-step chain launch at 94 x 95 with 1,024 chains,
 # from rocprofv3 FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors a
-# plain 1 GiB device copy gives those counters: 1,975 KiB x 2 + 6,304 KiB.
-TRAFFIC_100K_WINDOW = (1975 * 2 + 6304) * 1024
+# plain 1 GiB device copy gives those counters: 2,589 KiB x 2 + 6,304 KiB.
+TRAFFIC_100K_WINDOW = (2589 * 2 + 6304) * 1024
 FPGA_KSTEPS = 2938.25  # reference's published chain steps/s at 95 x 94 (kilo)
 
 
