@@ -105,7 +105,7 @@ int dev_alloc(bbs_ctx *ctx, DevBuf &b, size_t bytes) {
 
 /* log(tgamma(x)) as the reference computes it in float
  * (src/TreeScorer.hpp:123-125). Where tgammaf overflows (x > ~35.04) the
- * reference's prior is inf - inf = NaN; this build uses log-gamma in double
+ * reference's prior is inf - inf = NaN; this library uses log-gamma in double
  * instead (the only deliberate numeric deviation; same rule in the oracle). */
 float lgamma_term(float x) {
   float g = tgammaf(x);
@@ -267,11 +267,11 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
     return fail(nullptr, BBS_ERR_ARG, "params and data must not be NULL");
   if (n_genes < 2u || n_genes > BBS_MAX_GENES)
     return fail(nullptr, BBS_ERR_SIZE,
-                "this build supports 2 to %u genes; the input has %u",
+                "the library supports 2 to %u genes; the input has %u",
                 BBS_MAX_GENES, n_genes);
   if (n_cells < 1u || n_cells > BBS_MAX_CELLS)
     return fail(nullptr, BBS_ERR_SIZE,
-                "this build supports 1 to %u cells; the input has %u",
+                "the library supports 1 to %u cells; the input has %u",
                 BBS_MAX_CELLS, n_cells);
   const bbs_params &p = *params;
   if (!(p.alpha_mean > 0.f && p.alpha_mean < 1.f))

@@ -339,7 +339,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
-              "(chains shard across GPUs with one final argmax exchange; see DESIGN.md)",
+              "(chains shard across GPUs with one final argmax exchange)",
               file=sys.stderr)
         sys.exit(2)
 
