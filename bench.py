@@ -120,8 +120,12 @@ def scoring_bench(reps=5, peak=P_INT32_TOPS):
     n_trees = 1 << 20
     with B.Context(p, mut, kno, G) as ctx:
         ms, _ = ctx.bench_score_random(3, n_trees, reps, with_prior=False, n_head=0)
+        ms_v, _ = ctx.bench_score_random(3, n_trees, 3, with_prior=False, n_head=0,
+                                         valu_lanes=True)
     k_ms = float(np.median(ms))
     trees_s = n_trees / (k_ms / 1e3)
+    v_ms = float(np.median(ms_v))
+    v_trees_s = n_trees / (v_ms / 1e3)
     # The pair counts run on the matrix cores (int8 MFMA over 96 x 96 x 96
     # tiles, two matrices); what bounds the kernel is the float part of each
     # (cell, node) pair on the vector unit: 2 cvt, 2 sub, 4 mul, 3 add, 1 max.
@@ -139,6 +143,13 @@ def scoring_bench(reps=5, peak=P_INT32_TOPS):
                                    "frac": 2 * macs * trees_s / 1e12 / P_INT8_MFMA_TOPS}},
         "alg_int_tops": ALG_INT_OPS * trees_s / 1e12,
         "counted_bits_per_s": trees_s * 4 * N ** 3,
+        # the same trees through the vector-unit scorer (one tree per lane,
+        # AND + popcount over 32-bit words): its integer-op roofline
+        "valu_scorer": {"kernel": "score_lanes_kernel", "trees_per_s": v_trees_s,
+                        "kernel_ms": v_ms,
+                        "alg_int_tops": ALG_INT_OPS * v_trees_s / 1e12,
+                        "peak": peak,
+                        "frac": ALG_INT_OPS * v_trees_s / 1e12 / peak},
     }
 
 
