@@ -149,6 +149,10 @@ bool load_and_verify_args(int argc, char **argv, Cli &cli) {
                  "forget to set the -r parameter?"
               << std::endl;
     error = true;
+  } else if (*cli.n_chains == 0u) {
+    /* the reference would index the best of zero chains */
+    std::cerr << "Error: The number of markov chains (-r) must be at least 1." << std::endl;
+    error = true;
   }
   if (!cli.chain_length) {
     std::cerr << "Error: Missing markov chain length. Did you forget to set the "

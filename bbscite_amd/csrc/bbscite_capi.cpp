@@ -290,6 +290,8 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
                 "The sum of the move type probabilities is zero.");
   if (!(p.beta_jump_scaling_chi > 0.f))
     return fail(nullptr, BBS_ERR_ARG, "chi (-x) must be positive");
+  if (p.n_chains == 0u)
+    return fail(nullptr, BBS_ERR_ARG, "at least one chain (-r) is needed");
 
   int n_dev = 0;
   if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
@@ -955,7 +957,7 @@ int fill_job(const bbs_job &in, bbs::BatchJob &J, std::string &why) {
       p.prob_beta_change + p.prob_prune_n_reattach + p.prob_swap_nodes +
               p.prob_swap_subtrees ==
           0.f ||
-      p.n_chains > 65535u || !in.is_mutated || !in.is_known) {
+      p.n_chains == 0u || p.n_chains > 65535u || !in.is_mutated || !in.is_known) {
     why = "invalid parameters";
     return BBS_ERR_ARG;
   }
@@ -1676,6 +1678,7 @@ struct bbs_server {
   int device = -1;
   bbs_server_config cfg{};
   hipStream_t stream[BBS_MAX_STREAMS] = {};
+  hipStream_t retire_stream = nullptr; /* result copies of finished jobs only */
   std::vector<Lane *> lanes;
   std::map<uint32_t, double> ms_per_cost; /* layout -> ms per (step x cost unit) */
   uint32_t n_cus = 0, lds_per_cu = 0;
@@ -1814,11 +1817,14 @@ int srv_admit(bbs_server *s, bool &admitted) {
 
 /* The finished job's result: the chain-major first maximum of its chains
  * (src/Application.hpp:456), as bbs_get_best. */
-int srv_retire(bbs_server *s, hipStream_t st_, SrvJob *j, double t_done) {
+int srv_retire(bbs_server *s, SrvJob *j, double t_done) {
   const bbs::BatchJob &J = j->J;
   std::vector<float> sc(J.n_chains), bb(J.n_chains);
   std::vector<uint32_t> st(J.n_chains);
-  /* on the (idle) lane stream, so other lanes' slices keep running */
+  /* The job's last slice has ended (its lane's end event completed). The
+   * copies go on a stream that carries nothing else, so waiting for them
+   * never waits for a slice of another lane sharing the lane's stream. */
+  hipStream_t st_ = s->retire_stream;
   SHIP_TRY(s, hipMemcpyAsync(sc.data(), (float *)s->best_score.p + J.chain_offset,
                              J.n_chains * 4u, hipMemcpyDeviceToHost, st_));
   SHIP_TRY(s, hipMemcpyAsync(st.data(), (uint32_t *)s->best_step.p + J.chain_offset,
@@ -1971,7 +1977,7 @@ int srv_lane_complete(bbs_server *s, Lane *l) {
   }
   for (SrvJob *j : l->jobs) {
     if (j->started && j->steps_done >= j->n_steps) {
-      const int rc = srv_retire(s, l->stream, j, t_end);
+      const int rc = srv_retire(s, j, t_end);
       if (rc) return rc;
     } else {
       still.push_back(j);
@@ -2019,6 +2025,7 @@ void bbs_server_destroy(bbs_server *s) {
   for (DevBuf *d : bufs) d->release();
   for (int i = 0; i < BBS_MAX_STREAMS; i++)
     if (s->stream[i]) (void)hipStreamDestroy(s->stream[i]);
+  if (s->retire_stream) (void)hipStreamDestroy(s->retire_stream);
   delete s;
 }
 
@@ -2050,6 +2057,7 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
   if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
   for (uint32_t i = 0; i < c.n_streams && e == hipSuccess; i++)
     e = hipStreamCreateWithFlags(&s->stream[i], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->retire_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     bbs_server_destroy(s);
     return sfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
@@ -2120,6 +2128,15 @@ int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id) {
       j->J.n_cells > s->cellsA.cap) {
     delete j;
     return sfail(s, BBS_ERR_BUDGET, "job rejected: larger than the server's device memory budget");
+  }
+  /* a lane's job and item tables hold chain_cap entries; an empty server
+   * admits any job, so a larger one would overrun them */
+  if (j->J.n_chains > s->chain_cap) {
+    const uint32_t n = j->J.n_chains;
+    delete j;
+    return sfail(s, BBS_ERR_BUDGET,
+                 "job rejected: %u chains exceed the server's resident set of %u chains", n,
+                 s->chain_cap);
   }
   j->n_genes = G;
   j->n_steps = job->params.n_steps;

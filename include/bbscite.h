@@ -207,7 +207,8 @@ int bbs_eval_libm(bbs_ctx *ctx, const float *x, uint32_t n, float *exp_out,
 /* ---- batched form ---------------------------------------------------------
  * Many independent datasets ("jobs") run together: the (job, chain) items
  * of each register layout go out as one launch, largest work first, and the
- * layouts run side by side on up to four streams. Job j draws its random
+ * layouts run side by side on up to BBS_MAX_STREAMS (16) streams. Job j
+ * draws its random
  * stream from
  * (params.seed, job_tag, chain, step), so a job's results are identical to
  * a plain bbs_run of the same data and parameters when job_tag is 0,

@@ -552,6 +552,17 @@ void orc_run(const bbs_params *p, const orc_scorer *s, uint32_t job,
   free(res);
 }
 
+void orc_run_chains(const bbs_params *p, const orc_scorer *s, uint32_t job,
+                    uint32_t first, uint32_t count, int n_threads,
+                    orc_chain_result *results) {
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (uint32_t i = 0; i < count; i++)
+    orc_run_chain(p, s, job, first + i, first + i == 0u, NULL, NAN, &results[i], NULL);
+}
+
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   bbs_u32x4 r = bbs_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
   memcpy(out, r.x, sizeof(r.x));

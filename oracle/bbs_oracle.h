@@ -23,8 +23,15 @@
  *   beta prior                    : the formula of src/TreeScorer.hpp:122-126
  *       evaluated independently in double
  *   chain loop                    : restated test/Application.cpp:38-142
- *       (the planted 15x4 tree is found); Metropolis/best semantics checked
- *       against a Python restatement on small chains
+ *       (the planted 15x4 tree is found); on a fixture with duplicated
+ *       gene rows (many equal proposals) the global and per-chain maxima
+ *       equal an independent scan of the traces (first strictly greater
+ *       proposal in chain-major order, chain 0's step 0 seeding it); on
+ *       subtree-swap-only chains every acceptance decision equals one
+ *       recomputed from parent vectors (which node is the ancestor, the
+ *       nd(v)/nd(w) ratio, glibc expf) and every proposal's score equals
+ *       the score of the brute-force-moved tree
+ *       (tests/test_oracle_chain_rules.py)
  */
 #ifndef BBS_ORACLE_H
 #define BBS_ORACLE_H
@@ -121,6 +128,13 @@ void orc_run_chain(const bbs_params *p, const orc_scorer *s, uint32_t job,
 void orc_run(const bbs_params *p, const orc_scorer *s, uint32_t job,
              orc_tree *best_tree, float *best_score, uint32_t *best_chain,
              uint32_t *best_step);
+
+/* Chains [first, first + count) of one job, each as orc_run_chain (chain 0
+ * is the first chain), spread over n_threads host threads (0: all);
+ * results[i] is chain first + i. Replays whole batched or served runs. */
+void orc_run_chains(const bbs_params *p, const orc_scorer *s, uint32_t job,
+                    uint32_t first, uint32_t count, int n_threads,
+                    orc_chain_result *results);
 
 /* The shared random stream (bbscite_amd/csrc/bbs_rng.h), exported so that
  * tests can pin it: Philox4x32-10 blocks and raw moves. */

@@ -144,6 +144,10 @@ _SIGS = {
             C.POINTER(_u32),
         ],
     ),
+    "orc_run_chains": (
+        None,
+        [C.POINTER(Params), C.POINTER(Scorer), _u32, _u32, _u32, C.c_int, _P],
+    ),
     "orc_philox": (None, [_P, _P, _P]),
     "orc_draw_move": (RawMove, [C.c_uint64, _u32, _u32, _u32, _u32, MoveProbs]),
     "orc_normal": (C.c_double, [_u32, _u32]),
@@ -176,6 +180,15 @@ def lib():
             fn.argtypes = args
         _lib = l
     return _lib
+
+
+def use_library(path):
+    """Bind a different build of the same oracle (bench.py's CPU legs load
+    the host-tuned -O3 -march=native build this way)."""
+    global _lib, LIB_PATH
+    _lib = None
+    LIB_PATH = path
+    return lib()
 
 
 def _ptr(a):
@@ -263,6 +276,47 @@ def run(p: Params, s: Scorer, job=0):
     st = _u32()
     lib().orc_run(C.byref(p), C.byref(s), job, C.byref(t), C.byref(sc), C.byref(ch), C.byref(st))
     return t, sc.value, ch.value, st.value
+
+
+def host_threads():
+    """Host threads for parallel replays: the cores this process may use,
+    at most 16 (a GPU box's share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def run_chains(p: Params, s: Scorer, job=0, first=0, count=None, n_threads=None):
+    """Chains [first, first + count) in parallel; a ctypes array of
+    ChainResult (index i = chain first + i)."""
+    if count is None:
+        count = p.n_chains - first
+    res = (ChainResult * count)()
+    lib().orc_run_chains(C.byref(p), C.byref(s), job, first, count,
+                         n_threads or host_threads(), C.cast(res, C.c_void_p))
+    return res
+
+
+def run_chain_list(p: Params, s: Scorer, chains, job=0, n_threads=None):
+    """orc_run_chain for each chain of a list (any chain indices), on a pool
+    of host threads (ctypes releases the interpreter lock during the call);
+    ChainResult per chain, in list order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(n_threads or host_threads()) as ex:
+        return list(ex.map(lambda c: run_chain(p, s, c, c == 0, job=job), chains))
+
+
+def chain_major_best(results):
+    """The reference's global maximum (src/Application.hpp:456, P = 1) from
+    per-chain results of chains 0..n-1: chain 0's record seeds it, a later
+    chain replaces it only with a strictly greater score. Returns the chain
+    index (or None when no chain has a record)."""
+    best = None
+    for c, r in enumerate(results):
+        if r.best_step == 0xFFFFFFFF:
+            continue
+        if best is None or c == 0 or r.best_score > results[best].best_score:
+            best = c
+    return best
 
 
 def philox(ctr, key):

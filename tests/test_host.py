@@ -85,7 +85,7 @@ def test_uniform_draws_are_unbiased_enough():
     counts_type = np.zeros(4)
     for step in range(8000):
         m = O.draw_move(11, 0, 3, step, n, probs)
-        assert 0 <= m.raw_v <= n - 2 and 0 <= m.raw_w <= n - 1 and m.raw_w != m.raw_v
+        assert 0 <= m.raw_v <= n - 2 and 0 <= m.raw_w <= n - 2 and m.raw_w != m.raw_v
         assert 0.0 <= m.raw_descendant_of_v < 1.0 and 0.0 <= m.acceptance_level < 1.0
         counts_v[m.raw_v] += 1
         counts_type[m.move_type] += 1
@@ -262,3 +262,21 @@ def test_default_parameters_match_reference():
     assert math.isclose(p.beta_sd, 0.1, rel_tol=1e-6)
     assert math.isclose(p.gamma, 1.0)
     assert math.isclose(p.beta_jump_scaling_chi, 10.0)
+
+
+def test_zero_chains_are_rejected_before_the_device():
+    """bbs_create refuses n_chains = 0 with BBS_ERR_ARG (the reference would
+    take the best of zero chains); the CLI refuses -r 0."""
+    m = np.zeros((4, 3), np.int8)
+    mut, kno = B.encode_matrix(m)
+    with pytest.raises(B.BbsError) as e:
+        B.Context(B.default_params(n_chains=0, n_steps=10), mut, kno, 4)
+    assert e.value.code == _capi.ERR_ARG
+    assert "chain" in str(e.value)
+
+
+def test_cli_rejects_zero_chains(tmp_path):
+    f = tmp_path / "x.csv"
+    f.write_text("0 1\n1 0\n")
+    r = run_cli("-i", str(f), "-r", "0", "-l", "10")
+    assert r.returncode == 1 and "at least 1" in r.stderr
