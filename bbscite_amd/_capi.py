@@ -25,6 +25,7 @@ ERR_HIP = -3
 ERR_BUDGET = -4
 ERR_STATE = -5
 ERR_NODEV = -6
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 MOVE_CHANGE_BETA = 0
 MOVE_PRUNE_REATTACH = 1
@@ -170,6 +171,7 @@ _SIGS = {
     "bbs_get_chain_states": (C.c_int, [_P, _P, _P]),
     "bbs_get_trace": (C.c_int, [_P, _P]),
     "bbs_score_trees": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
+    "bbs_score_trees_noalloc": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
     "bbs_bench_score_random": (C.c_int, [_P, _u64, _u32, _u32, _u32, _P, _P, _u32]),
     "bbs_random_trees_device": (C.c_int, [_P, _u64, _u32, _u32, _P, _P]),
     "bbs_eval_libm": (C.c_int, [_P, _P, _u32, _P, _P, _P]),
@@ -482,7 +484,7 @@ class Context:
         return out
 
     def score_trees(self, dm_rows, betas=None, with_prior=True, counts=False,
-                    cell_lanes=False, valu_lanes=False):
+                    cell_lanes=False, valu_lanes=False, no_alloc=False):
         dm_rows = np.ascontiguousarray(dm_rows, np.uint64)
         T = dm_rows.shape[0]
         ll = np.zeros(T, np.float32)
@@ -490,7 +492,8 @@ class Context:
         cnt = np.zeros((T, self.n_cells, 4), np.uint8) if counts else None
         flags = ((0 if with_prior else SCORE_NO_PRIOR) | (SCORE_CELL_LANES if cell_lanes else 0)
                  | (SCORE_VALU_LANES if valu_lanes else 0))
-        self._c(lib().bbs_score_trees(self.h, _ptr(dm_rows), T, _ptr(b), flags, _ptr(ll), _ptr(cnt)))
+        fn = lib().bbs_score_trees_noalloc if no_alloc else lib().bbs_score_trees
+        self._c(fn(self.h, _ptr(dm_rows), T, _ptr(b), flags, _ptr(ll), _ptr(cnt)))
         return (ll, cnt) if counts else ll
 
     def bench_score_random(self, seed, n_trees, reps, with_prior=False, n_head=0,

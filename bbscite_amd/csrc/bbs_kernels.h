@@ -117,6 +117,11 @@ hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first
  * nodes in this layout, 0 if the layout has no incremental scorer */
 uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes);
 
+/* Device address of this device's staging array for bbs_score_trees_noalloc
+ * (BBS_NOALLOC_SCRATCH_BYTES), looked up once per device with the device
+ * current and kept; later calls return the kept address. */
+hipError_t noalloc_scratch(int device, void **ptr);
+
 hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
                             uint32_t iters, hipStream_t stream);
 hipError_t launch_libm(const float *x, uint32_t n, float *e, float *l, float *l1m,

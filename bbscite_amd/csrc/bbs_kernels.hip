@@ -15,6 +15,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "bbs_device.h"
 #include "bbs_kernels.h"
 
@@ -990,6 +992,35 @@ __device__ void random_tree_wave(const TreeGenArgs &a, uint32_t tree) {
 __global__ __launch_bounds__(64) void random_trees_kernel(TreeGenArgs a) {
   for (uint32_t tree = blockIdx.x; tree < a.n_trees; tree += gridDim.x)
     random_tree_wave(a, tree);
+}
+
+/* ------------------------------------------------ allocation-free staging */
+
+/* The staging of bbs_score_trees_noalloc: one fixed array per GPU, part of
+ * this code object, so a call needs no device allocation. In host code its
+ * name is only a stand-in: kernels and copies use the address that
+ * hipGetSymbolAddress returns for the current device. */
+static __device__ __align__(256) unsigned char scratch[BBS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+hipError_t noalloc_scratch(int device, void **ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
 }
 
 /* ------------------------------------------------------ VALU peak probe */

@@ -63,6 +63,8 @@ struct bbs_ctx {
   bool begun = false;        /* bests reset, steps counted from 0     */
   uint32_t steps_done = 0;
   bool ran = false;
+  void *scratch = nullptr;   /* this device's bbs_score_trees_noalloc staging */
+  std::string scratch_err;   /* why scratch is null                   */
   std::string err;
 };
 
@@ -377,6 +379,16 @@ int bbs_create(const bbs_params *params, uint32_t n_cells, uint32_t n_genes,
     g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
     bbs_destroy(ctx);
     return BBS_ERR_HIP;
+  }
+  /* the allocation-free scorer's staging address on this device (looked up
+   * once per device; only bbs_score_trees_noalloc needs it) */
+  {
+    const hipError_t es = bbs::noalloc_scratch(dev, &ctx->scratch);
+    if (es != hipSuccess) {
+      ctx->scratch = nullptr;
+      ctx->scratch_err = hipGetErrorString(es);
+      (void)hipGetLastError();
+    }
   }
   const size_t rows = (size_t)p.n_chains * ctx->n_nodes * 2u * sizeof(uint64_t);
   const size_t per_chain = (size_t)p.n_chains * sizeof(float);
@@ -713,6 +725,57 @@ int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
   d_counts.release();
   if (e != hipSuccess)
     return fail(ctx, BBS_ERR_HIP, "bbs_score_trees: %s", hipGetErrorString(e));
+  return BBS_OK;
+}
+
+int bbs_score_trees_noalloc(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
+                            const float *betas, uint32_t flags, float *ll,
+                            uint8_t *counts) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (n_trees == 0u) return BBS_OK;
+  if (!dm_rows || !ll) return fail(ctx, BBS_ERR_ARG, "dm_rows and ll must not be NULL");
+  /* staging layout in the scratch array: rows, betas, scores, counts, each
+   * rounded up to 256 bytes */
+  auto up = [](uint64_t x) { return (x + 255u) & ~(uint64_t)255u; };
+  const uint64_t rows = (uint64_t)n_trees * ctx->n_nodes * 16u;
+  const uint64_t o_betas = up(rows);
+  const uint64_t o_ll = o_betas + (betas ? up(4ull * n_trees) : 0u);
+  const uint64_t o_counts = o_ll + up(4ull * n_trees);
+  const uint64_t need = o_counts + (counts ? up(4ull * n_trees * ctx->n_cells) : 0u);
+  if (need > BBS_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, BBS_ERR_SIZE,
+                "bbs_score_trees_noalloc: %u trees need %llu bytes of staging, more than "
+                "its fixed %u bytes; score them in smaller calls",
+                n_trees, (unsigned long long)need, (unsigned)BBS_NOALLOC_SCRATCH_BYTES);
+  if (!ctx->scratch)
+    return fail(ctx, BBS_ERR_HIP, "bbs_score_trees_noalloc: no staging address on this device (%s)",
+                ctx->scratch_err.c_str());
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  char *base = (char *)ctx->scratch;
+  bbs::ScoreArgs s{};
+  score_args(ctx, s, flags);
+  s.dm = (const uint64_t *)base;
+  s.betas = betas ? (const float *)(base + o_betas) : nullptr;
+  s.ll = (float *)(base + o_ll);
+  s.counts = counts ? (uint8_t *)(base + o_counts) : nullptr;
+  s.n_trees = n_trees;
+  hipError_t e = hipMemcpyAsync(base, dm_rows, rows, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && betas)
+    e = hipMemcpyAsync(base + o_betas, betas, 4ull * n_trees, hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess)
+    e = bbs::launch_score(s, counts != nullptr, !(flags & BBS_SCORE_VALU_LANES), ctx->stream,
+                          (flags & BBS_SCORE_CELL_LANES) ? nullptr
+                                                         : (const uint32_t *)ctx->cells32.p);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ll, base + o_ll, 4ull * n_trees, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && counts)
+    e = hipMemcpyAsync(counts, base + o_counts, 4ull * n_trees * ctx->n_cells,
+                       hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = es;
+  if (e != hipSuccess)
+    return fail(ctx, BBS_ERR_HIP, "bbs_score_trees_noalloc: %s", hipGetErrorString(e));
   return BBS_OK;
 }
 

@@ -174,6 +174,26 @@ int bbs_score_trees(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
                     uint8_t *counts);
 
 /*
+ * No counterpart in the reference: an allocation-free bbs_score_trees for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Arguments,
+ * results and errors are bbs_score_trees's; the per-call device staging
+ * (the trees' rows, betas, scores and counts) lives in one fixed device
+ * array of BBS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends,
+ * instead of buffers allocated per call, and the call makes no hipMalloc or
+ * hipFree. The staging of one call, with each part rounded up to 256 bytes,
+ * is 16 n_trees n_nodes (rows) + 4 n_trees (betas, when given) + 4 n_trees
+ * (scores) + 4 n_trees n_cells (counts, when requested) bytes; a call that
+ * needs more returns BBS_ERR_SIZE. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several threads or contexts)
+ * race.
+ */
+#define BBS_NOALLOC_SCRATCH_BYTES (16u << 20)
+int bbs_score_trees_noalloc(bbs_ctx *ctx, const uint64_t *dm_rows, uint32_t n_trees,
+                            const float *betas, uint32_t flags, float *ll,
+                            uint8_t *counts);
+
+/*
  * Bulk scoring with device-resident inputs (the config of
  * cpu_scoring_benchmark/main.cpp:168-212: random trees scored against one
  * matrix): n_trees uniform random trees are drawn on the device from seed
