@@ -110,6 +110,7 @@ class BatchConfig(C.Structure):
         ("persistent", C.c_uint32),
         ("scoring", C.c_uint32),
         ("graph", C.c_uint32),
+        ("two_wave_layouts", C.c_uint32),
     ]
 
 
@@ -123,6 +124,7 @@ class ServerConfig(C.Structure):
         ("max_resident_chains", C.c_uint32),
         ("waves_per_group", C.c_uint32),
         ("n_streams", C.c_uint32),
+        ("two_wave_layouts", C.c_uint32),
     ]
 
 
@@ -557,11 +559,21 @@ def _bcheck(rc, b=None):
         raise BbsError(rc, msg.decode() if msg else "")
 
 
-def batch_config(waves_per_group=None, n_streams=None, order=None, scoring=None, graph=None):
+def layout_bit(ns, cs, w):
+    """BBS_LAYOUT_BIT: the layout's bit in two_wave_layouts."""
+    return 1 << ((ns - 1) * 12 + (cs - 1) * 4 + (w - 1))
+
+
+def batch_config(waves_per_group=None, n_streams=None, order=None, scoring=None, graph=None,
+                 two_wave_layouts=None):
     """scoring: None/"auto" (incremental where the tables fit) or "full";
-    graph: capture the run in a hipGraph on the first run, replay later."""
+    graph: capture the run in a hipGraph on the first run, replay later;
+    two_wave_layouts: layout bits (layout_bit) of the two-waves-per-SIMD
+    register budget."""
     c = BatchConfig()
     lib().bbs_batch_config_default(C.byref(c))
+    if two_wave_layouts is not None:
+        c.two_wave_layouts = two_wave_layouts
     if graph is not None:
         c.graph = 1 if graph else 0
     if scoring is not None:

@@ -316,18 +316,39 @@ __device__ __forceinline__ float sum_padded(const float *x) {
 }
 
 /* The pair-score table of the current tree in LDS, node-major per cell:
- * the scores of cell c (the cell a lane scores in one slot: 64 cells, 96
- * with the paired half slot, or 128) over all nodes form one row of
- * `stride` floats. stride = 8 ceil(n_nodes / 8) + 4 is 4 times an odd
- * number, so the 16-byte reads of 16 consecutive lanes (one row each) hit
- * distinct banks; a row also covers the padding nodes the 4-node groups of
- * the scan read. */
+ * the scores of cell c over all nodes form row c of `stride` floats.
+ * stride = 8 ceil(n_nodes / 8) + 4 is 4 times an odd number, so the 16-byte
+ * reads of 16 consecutive lanes (one row each) hit distinct banks; a row also
+ * covers the padding nodes the 4-node groups of the scan read. There are
+ * rows only for the dataset's n_cells cells. A lane without a cell (cell >=
+ * n_cells, up to the layout's slots) reads and writes a spare row that
+ * starts 4 (cell - n_cells) floats past the last cell's row: its values are
+ * never used, and the spare rows of different lanes start at different
+ * 16-byte offsets, so their writes do not collide. */
 struct TableView {
   float *t;
   uint32_t stride;
-  __device__ static uint32_t stride_for(uint32_t n_nodes) { return ((n_nodes + 7u) & ~7u) + 4u; }
-  __device__ static uint32_t pen_len(uint32_t n_nodes) { return stride_for(n_nodes) + 4u; }
+  uint32_t n_cells;
+  __host__ __device__ static uint32_t stride_for(uint32_t n_nodes) {
+    return ((n_nodes + 7u) & ~7u) + 4u;
+  }
+  __host__ __device__ static uint32_t pen_len(uint32_t n_nodes) { return stride_for(n_nodes) + 4u; }
+  /* floats past a row's start or S.pen that the unchanged-node scan may
+   * read: 64 NS + 8 for NS = 1 node slot up to 64 nodes, else 2 */
+  __host__ __device__ static uint32_t scan_reach(uint32_t n_nodes) {
+    return n_nodes <= 64u ? 72u : 136u;
+  }
+  /* floats of a table for a layout of `slots` cells (64, 96 or 128) */
+  __host__ __device__ static uint32_t floats_for(uint32_t n_nodes, uint32_t n_cells,
+                                                 uint32_t slots) {
+    const uint32_t st = stride_for(n_nodes);
+    return n_cells < slots ? n_cells * st + 4u * (slots - n_cells - 1u) + st : slots * st;
+  }
+  __device__ float *row(uint32_t cell) const {
+    return cell < n_cells ? t + cell * stride : t + n_cells * stride + 4u * (cell - n_cells);
+  }
 };
+
 template <int CS>
 struct TableRows {
   static constexpr uint32_t n = CS == 1 ? 64u : CS == 3 ? 96u : 128u;
@@ -421,8 +442,11 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
   static_assert(!COUNTS || CS != 3, "counts need full cell slots");
   constexpr int SL = Slots<CS>::n;
   const uint32_t lane = lane_id();
-  float *const trow0 = TAB ? tab.t + lane * tab.stride : nullptr;
-  float *const trowh = TAB ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
+  float *trow[2] = {nullptr, nullptr};
+  if (TAB) {
+    trow[0] = tab.row(slot_cell<CS>(0));
+    trow[1] = tab.row(slot_cell<CS>(1));
+  }
   nan0[0] = nan0[1] = false;
   /* one node for the full slots */
   auto full = [&](const uint4 &A, uint32_t node, bool first) {
@@ -431,7 +455,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
       uint32_t n01, n11;
       pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
       const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      if (TAB) trow0[64u * (uint32_t)j * tab.stride + node] = s;
+      if (TAB) trow[j][node] = s;
       if (first) {
         best[j] = s;
         nan0[j] = s != s;
@@ -456,7 +480,7 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
     constexpr int H = CS == 3 ? 1 : 0;
     pair_counts<W>(c.km[H], c.knm[H], A, n01, n11);
     const float s = pair_score(n01, n11, c.fa[H], c.fb[H], L);
-    if (TAB) trowh[node] = s;
+    if (TAB) trow[1][node] = s;
     if (first) {
       best[1] = s;
       nan0[1] = lane < 32u && s != s;
@@ -593,7 +617,7 @@ __device__ __forceinline__ float mfma_maxima(const MfmaCells<CS, W> &mc, uint32_
       if (SUM && lane < 32u) S.sums[cell] = 0.0f;
       continue;
     }
-    float *const trow = tab.t + cell * tab.stride;
+    float *const trow = tab.row(cell);
     float best = ninf;
     bool nan0 = false;
 #pragma unroll
@@ -646,7 +670,7 @@ template <int NS, int CS, int W, bool COUNTS, bool TAB = false>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
            uint32_t n_cells, const LogProbs &L, WaveLds &S,
-           uint32_t (&arg01)[2], uint32_t (&arg11)[2], TableView tab = TableView{nullptr, 0u}) {
+           uint32_t (&arg01)[2], uint32_t (&arg11)[2], TableView tab = TableView{nullptr, 0u, 0u}) {
   store_rows<NS>(dm, n_nodes, S);
   float best[2];
   bool nan0[2];
@@ -681,8 +705,11 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
   constexpr int SL = Slots<CS>::n;
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
-  float *const trow0 = WRITE ? tab.t + lane * tab.stride : nullptr;
-  float *const trowh = WRITE ? tab.t + (64u + (lane & 31u)) * tab.stride : nullptr;
+  float *trow[2] = {nullptr, nullptr};
+  if (WRITE) {
+    trow[0] = tab.row(slot_cell<CS>(0));
+    trow[1] = tab.row(slot_cell<CS>(1));
+  }
   uint64_t m[NS];
 #pragma unroll
   for (int k = 0; k < NS; k++) m[k] = changed[k];
@@ -726,8 +753,8 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
       pair_counts<W>(c.km[j], c.knm[j], p.Ab, n01, n11);
       const float sb = pair_score(n01, n11, c.fa[j], c.fb[j], L);
       if (WRITE) {
-        trow0[64u * (uint32_t)j * tab.stride + p.a] = sa;
-        trow0[64u * (uint32_t)j * tab.stride + p.b] = sb;
+        trow[j][p.a] = sa;
+        trow[j][p.b] = sb;
       } else {
         R[j] = vmax(vmax(R[j], sa), sb);
       }
@@ -737,7 +764,7 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
       pair_counts<W>(c.km[1], c.knm[1], p.Ah, n01, n11);
       const float sh = pair_score(n01, n11, c.fa[1], c.fb[1], L);
       if (WRITE)
-        trowh[hi_half ? p.b : p.a] = sh;
+        trow[1][hi_half ? p.b : p.a] = sh;
       else
         R[1] = vmax(R[1], sh);
     }
@@ -794,18 +821,17 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
   const uint32_t gh = (n_nodes + 7u) >> 3; /* 4-node groups per half */
-  const float *const r0 = tab.t + lane * tab.stride;
-  const float *const rh =
-      tab.t + (64u + (lane & 31u)) * tab.stride + (hi_half ? 4u * gh : 0u);
+  const float *const rj[2] = {tab.row(slot_cell<CS>(0)), tab.row(slot_cell<CS>(1))};
+  const float *const rh = rj[1] + (hi_half ? 4u * gh : 0u);
   const float *const qh = S.pen + (hi_half ? 4u * gh : 0u);
   if (!node0_changed) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
-      const float x = r0[64u * (uint32_t)j * tab.stride];
+      const float x = rj[j][0];
       nan0[j] = x != x;
     }
     if constexpr (CS == 3) {
-      const float x = tab.t[(64u + (lane & 31u)) * tab.stride];
+      const float x = rj[1][0];
       if (hi_half == 0u) nan0[1] = x != x;
     }
   }
@@ -820,9 +846,8 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
   auto rd = [&](uint32_t g, Grp &b, Grp &o) {
 #pragma unroll
     for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
-      b.x[j][0] = *reinterpret_cast<const float4 *>(r0 + 64u * (uint32_t)j * tab.stride + 4u * g);
-      b.x[j][1] =
-          *reinterpret_cast<const float4 *>(r0 + 64u * (uint32_t)j * tab.stride + 4u * g + 4u);
+      b.x[j][0] = *reinterpret_cast<const float4 *>(rj[j] + 4u * g);
+      b.x[j][1] = *reinterpret_cast<const float4 *>(rj[j] + 4u * g + 4u);
     }
     b.q[0] = *reinterpret_cast<const float4 *>(S.pen + 4u * g);
     b.q[1] = *reinterpret_cast<const float4 *>(S.pen + 4u * g + 4u);
@@ -858,11 +883,14 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
     if constexpr (CS == 3) mx4(U[1], b.xh, b.qhv);
   };
   const uint32_t g4 = (n_nodes + 3u) >> 2; /* 4-node groups of the full slots */
-  /* The reads of the next step are issued unconditionally (past the end
-   * they repeat the last step's groups and are not used): with the same
-   * number of reads in flight on every path, each wait is for exactly the
-   * reads it needs. */
-  const uint32_t gl = (g4 - 1u) & ~1u;
+  /* The reads of the next step are issued unconditionally and unclamped
+   * (past the end they read beyond the last group and are not used): with
+   * the same number of reads in flight on every path, each wait is for
+   * exactly the reads it needs, and every address is a base plus an
+   * immediate offset, so no per-group address stays live across steps. The
+   * furthest read ends 64 NS + 8 floats past a row's start or S.pen; the
+   * slab keeps that much LDS after the last row and after S.pen
+   * (scan_reach). */
   Grp A, B;
 #pragma unroll
   for (int j = 0; j < 2; j++) {
@@ -876,10 +904,10 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
 #pragma unroll
   for (int it = 0; it < NS * 4; it++) {
     const uint32_t g = 4u * (uint32_t)it;
-    rd(g + 2u < g4 ? g + 2u : gl, B, A);
+    rd(g + 2u, B, A);
     use(A);
     if (g + 2u >= g4) break;
-    rd(g + 4u < g4 ? g + 4u : gl, A, B);
+    rd(g + 4u, A, B);
     use(B);
     if (g + 4u >= g4) break;
   }

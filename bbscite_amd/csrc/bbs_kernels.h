@@ -109,13 +109,14 @@ hipError_t launch_random_trees(const TreeGenArgs &a, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t stream);
 /* items [first, first + count) all have the given shape code; wave_floats
  * 0 for the full scorer, else the LDS floats of each wave's incremental
- * slab (batch_wave_floats of the largest tree in the range) */
+ * slab (the largest batch_wave_floats of the range's datasets);
+ * waves_per_simd 1 or 2 picks the incremental scorer's register budget */
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
                               uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
-                              hipStream_t stream);
-/* LDS floats of an incremental chain wave for trees of up to max_nodes
- * nodes in this layout, 0 if the layout has no incremental scorer */
-uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes);
+                              uint32_t waves_per_simd, hipStream_t stream);
+/* LDS floats of an incremental chain wave for datasets of up to max_nodes
+ * nodes and max_cells cells in this layout */
+uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes, uint32_t max_cells);
 
 /* Device address of this device's staging array for bbs_score_trees_noalloc
  * (BBS_NOALLOC_SCRATCH_BYTES), looked up once per device with the device

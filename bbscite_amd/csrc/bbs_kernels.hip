@@ -191,7 +191,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   MfmaCells<CS, W> mc;
   if (use_mfma) load_mfma_cells<CS, W>(a.cells, a.n_cells, mc);
 
-  uint64_t am[NS][NS], dm[NS][NS], bam[NS][NS], bdm[NS][NS];
+  uint64_t am[NS][NS], dm[NS][NS];
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     const uint32_t x = lane + 64u * (uint32_t)k;
@@ -218,26 +218,19 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       t = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, L, lds, tab);
     } else {
       t = score_tree<NS, CS, W, false, INCR>(dm, cells, N, a.n_cells, L, lds, unused01,
-                                             unused11, INCR ? tab : TableView{nullptr, 0u});
+                                             unused11, INCR ? tab : TableView{nullptr, 0u, 0u});
     }
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
     cur_score = *a.score;
   }
 
+  /* the best tree's rows stay in memory: a new maximum overwrites them
+   * there at once (new maxima are rare after the first steps, and the rows
+   * then cost no registers) */
   float best_score = *a.best_score;
   float best_beta = *a.best_beta;
   uint32_t best_step = *a.best_step;
-  const bool have_best = best_step != 0xffffffffu;
-#pragma unroll
-  for (int k = 0; k < NS; k++) {
-    const uint32_t x = lane + 64u * (uint32_t)k;
-#pragma unroll
-    for (int wd = 0; wd < NS; wd++) {
-      bam[k][wd] = (have_best && x < N) ? a.best_am[2u * x + (uint32_t)wd] : 0ull;
-      bdm[k][wd] = (have_best && x < N) ? a.best_dm[2u * x + (uint32_t)wd] : 0ull;
-    }
-  }
 
   /* the Philox blocks of 64 consecutive steps, one step per lane,
    * computed on the vector unit; each step reads its 8 words by readlane */
@@ -367,7 +360,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       } else {
         prop_raw = score_tree<NS, CS, W, false, INCR>(pdm, cells, N, a.n_cells, Lp, lds,
                                                       unused01, unused11,
-                                                      INCR ? tab : TableView{nullptr, 0u});
+                                                      INCR ? tab : TableView{nullptr, 0u, 0u});
       }
     }
     const float prop_score = prop_raw + prop_prior;
@@ -399,12 +392,20 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       best_beta = prop_beta;
       best_step = step;
 #pragma unroll
-      for (int k = 0; k < NS; k++)
+      for (int k = 0; k < NS; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+        if (x < N) {
 #pragma unroll
-        for (int wd = 0; wd < NS; wd++) {
-          bam[k][wd] = pam[k][wd];
-          bdm[k][wd] = pdm[k][wd];
+          for (int wd = 0; wd < NS; wd++) {
+            a.best_am[2u * x + (uint32_t)wd] = pam[k][wd];
+            a.best_dm[2u * x + (uint32_t)wd] = pdm[k][wd];
+          }
+          if (NS == 1) {
+            a.best_am[2u * x + 1u] = 0ull;
+            a.best_dm[2u * x + 1u] = 0ull;
+          }
         }
+      }
     }
     if (INCR && m.type == BBS_MOVE_CHANGE_BETA && !accept) {
       /* the table holds the rejected beta's scores: back to the current
@@ -448,14 +449,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       for (int wd = 0; wd < NS; wd++) {
         a.am[2u * x + (uint32_t)wd] = am[k][wd];
         a.dm[2u * x + (uint32_t)wd] = dm[k][wd];
-        a.best_am[2u * x + (uint32_t)wd] = bam[k][wd];
-        a.best_dm[2u * x + (uint32_t)wd] = bdm[k][wd];
       }
       if (NS == 1) {
         a.am[2u * x + 1u] = 0ull;
         a.dm[2u * x + 1u] = 0ull;
-        a.best_am[2u * x + 1u] = 0ull;
-        a.best_dm[2u * x + 1u] = 0ull;
       }
     }
   }
@@ -472,17 +469,27 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
 
 /* One dataset per launch, one wavefront per chain (the drop-in form). */
 /* Dynamic LDS of an incremental chain wave, in floats: the pair-score
- * table, then the tree rows, the per-cell sums and the node penalties. */
+ * table (a row per cell of the dataset and the spare rows), then the tree rows
+ * (16 bytes per node), the per-cell sums of the layout's cell slots and the
+ * node penalties. Every part starts on a 16-byte boundary and the total is a
+ * multiple of 4 floats, so consecutive slabs stay aligned. The scan's unused
+ * reads past the end (TableView::scan_reach) stay inside the slab: after the
+ * last table row come at least 4 n_nodes + 64 floats, and the penalties get
+ * at least scan_reach floats. */
 struct ChainLds {
   uint32_t table, rows, sums, pen, total;
-  __host__ __device__ static ChainLds of(uint32_t n_nodes, int cs) {
-    const uint32_t stride = ((n_nodes + 7u) & ~7u) + 4u;
+  __host__ __device__ static ChainLds of(uint32_t n_nodes, uint32_t n_cells, int cs) {
+    const uint32_t stride = TableView::stride_for(n_nodes);
+    const uint32_t slots = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
+    const uint32_t pen_floats = TableView::pen_len(n_nodes) > TableView::scan_reach(n_nodes)
+                                    ? TableView::pen_len(n_nodes)
+                                    : TableView::scan_reach(n_nodes);
     ChainLds l;
     l.table = 0u;
-    l.rows = (cs == 1 ? 64u : cs == 3 ? 96u : 128u) * stride;
+    l.rows = TableView::floats_for(n_nodes, n_cells, slots);
     l.sums = l.rows + 4u * n_nodes;
-    l.pen = l.sums + (cs == 1 ? 64u : cs == 3 ? 96u : 128u);
-    l.total = l.pen + stride + 4u;
+    l.pen = l.sums + slots;
+    l.total = l.pen + pen_floats;
     return l;
   }
 };
@@ -521,14 +528,14 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.prior = a.prior;
   v.probs = a.probs;
   if constexpr (INCR) {
-    const ChainLds l = ChainLds::of(a.n_nodes, CS);
+    const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
     run_chain<NS, CS, W, TRACE, true, true>(
-        v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes)});
+        v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes), a.n_cells});
   } else {
     __shared__ WaveLdsStore store;
     WaveLds lds = store.view();
-    run_chain<NS, CS, W, TRACE, false, false>(v, lds, TableView{nullptr, 0u});
+    run_chain<NS, CS, W, TRACE, false, false>(v, lds, TableView{nullptr, 0u, 0u});
   }
 }
 
@@ -539,11 +546,15 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
  * register layout (shape) are launched together, largest work first, so
  * the hardware dispatcher hands the next item to whichever SIMD frees up
  * first. waves_per_group wavefronts share a workgroup and run independent
- * items (run_chain has no workgroup barrier).
+ * items (run_chain has no workgroup barrier). WPS is the register budget:
+ * 1 lets the compiler use every register for one wave per SIMD (the most
+ * instruction-level parallelism); 2 holds a wave to 256 registers so that
+ * two share a SIMD (latency hidden by the other wave, at the price of a
+ * few spills in the larger layouts). Results are identical.
  */
-template <int NS, int CS, int W, bool INCR>
-__global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t first,
-                                                          uint32_t count, uint32_t wave_floats) {
+template <int NS, int CS, int W, bool INCR, int WPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS))) void
+batch_items_kernel(BatchArgs a, uint32_t first, uint32_t count, uint32_t wave_floats) {
   extern __shared__ __align__(16) float dyn[]; /* INCR: one ChainLds slab per wave */
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t k = blockIdx.x * (blockDim.x >> 6) + wave;
@@ -581,14 +592,14 @@ __global__ __launch_bounds__(256) void batch_items_kernel(BatchArgs a, uint32_t 
   v.probs = J.probs;
   if constexpr (INCR) {
     float *slab = dyn + wave * wave_floats;
-    const ChainLds l = ChainLds::of(J.n_nodes, CS);
+    const ChainLds l = ChainLds::of(J.n_nodes, J.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
-    run_chain<NS, CS, W, false, true, false>(v, lds,
-                                      TableView{slab + l.table, TableView::stride_for(J.n_nodes)});
+    run_chain<NS, CS, W, false, true, false>(
+        v, lds, TableView{slab + l.table, TableView::stride_for(J.n_nodes), J.n_cells});
   } else {
     __shared__ WaveLdsStore lds_all[4];
     WaveLds lds = lds_all[wave].view();
-    run_chain<NS, CS, W, false, false, false>(v, lds, TableView{nullptr, 0u});
+    run_chain<NS, CS, W, false, false, false>(v, lds, TableView{nullptr, 0u, 0u});
   }
 }
 
@@ -1090,7 +1101,7 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
 uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
   int ns, cs, w;
   shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
-  const uint32_t bytes = 4u * ChainLds::of(n_nodes, cs).total;
+  const uint32_t bytes = 4u * ChainLds::of(n_nodes, n_cells, cs).total;
   return bytes <= kMaxGroupLds ? bytes : 0u;
 }
 
@@ -1120,7 +1131,7 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
                         hipStream_t stream) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
-  if (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, cs).total)
+  if (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, a.n_cells, cs).total)
     return hipErrorInvalidValue;
 #define X(NS_, CS_, W_)                                                        \
   if (ns == NS_ && cs == CS_ && w == W_)                                       \
@@ -1184,34 +1195,44 @@ hipError_t launch_score(const ScoreArgs &a, bool counts, bool mfma, hipStream_t 
   return hipErrorInvalidValue;
 }
 
-uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes) {
+uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes, uint32_t max_cells) {
   const int cs = (int)(shape / 10u % 10u);
-  return ChainLds::of(max_nodes, cs).total;
+  return ChainLds::of(max_nodes, max_cells, cs).total;
 }
 
 template <int NS, int CS, int W>
 static hipError_t launch_items_shape(const BatchArgs &a, uint32_t first, uint32_t count,
-                                     uint32_t wpg, uint32_t wave_floats, hipStream_t stream) {
+                                     uint32_t wpg, uint32_t wave_floats, uint32_t wps,
+                                     hipStream_t stream) {
   const uint32_t groups = (count + wpg - 1u) / wpg;
   if (wave_floats) {
-    hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true>), dim3(groups), dim3(64u * wpg),
-                       4u * wave_floats * wpg, stream, a, first, count, wave_floats);
+    if (wps == 2u)
+      hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true, 2>), dim3(groups),
+                         dim3(64u * wpg), 4u * wave_floats * wpg, stream, a, first, count,
+                         wave_floats);
+    else
+      hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, true, 1>), dim3(groups),
+                         dim3(64u * wpg), 4u * wave_floats * wpg, stream, a, first, count,
+                         wave_floats);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, false>), dim3(groups), dim3(64u * wpg), 0,
-                     stream, a, first, count, 0u);
+  /* the full scorer fits two waves per SIMD in every layout as it is */
+  hipLaunchKernelGGL((batch_items_kernel<NS, CS, W, false, 1>), dim3(groups), dim3(64u * wpg),
+                     0, stream, a, first, count, 0u);
   return hipGetLastError();
 }
 
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
                               uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
-                              hipStream_t stream) {
+                              uint32_t waves_per_simd, hipStream_t stream) {
   if (count == 0u) return hipSuccess;
   if (waves_per_group < 1u || waves_per_group > 4u) return hipErrorInvalidValue;
+  if (waves_per_simd < 1u || waves_per_simd > 2u) return hipErrorInvalidValue;
   if (4ull * wave_floats * waves_per_group > kMaxGroupLds) return hipErrorInvalidValue;
 #define XB(NS_, CS_, W_)                                                       \
   if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_))                        \
-    return launch_items_shape<NS_, CS_, W_>(a, first, count, waves_per_group, wave_floats, stream);
+    return launch_items_shape<NS_, CS_, W_>(a, first, count, waves_per_group, wave_floats,  \
+                                            waves_per_simd, stream);
   BBS_SHAPES(XB)
 #undef XB
   return hipErrorInvalidValue;

@@ -951,10 +951,11 @@ struct bbs_batch {
   std::vector<uint32_t> items;
   struct Range {
     uint32_t shape, first, count;
-    uint32_t max_nodes; /* largest tree of the range (LDS of its slabs) */
+    uint32_t max_floats; /* largest incremental slab of its datasets    */
     double work;
   };
-  std::vector<Range> ranges; /* items of one shape, heaviest range first */
+  /* items of one shape, heaviest range first */
+  std::vector<Range> ranges;
   bbs_batch_config cfg{};
   uint32_t n_cus = 0;
   uint64_t total_steps = 0;
@@ -1067,6 +1068,17 @@ double step_cost(const bbs::BatchJob &J) {
   return (double)J.n_nodes * slots * (4.0 * w + 12.0) + 600.0;
 }
 
+/* Register budget of a layout's incremental scorer: waves per SIMD. */
+uint32_t layout_wps(uint32_t shape, uint32_t two_wave_layouts) {
+  const uint32_t ns = shape / 100u, cs = shape / 10u % 10u, w = shape % 10u;
+  return (two_wave_layouts & BBS_LAYOUT_BIT(ns, cs, w)) ? 2u : 1u;
+}
+
+/* LDS floats of a job's incremental slab. */
+uint32_t job_floats(const bbs::BatchJob &J) {
+  return bbs::batch_wave_floats(J.shape, J.n_nodes, J.n_cells);
+}
+
 void order_items(bbs_batch *b) {
   b->items.clear();
   b->ranges.clear();
@@ -1076,19 +1088,21 @@ void order_items(bbs_batch *b) {
     const auto &A = b->jobs[x];
     return step_cost(A) * (A.step_end - A.step_begin);
   };
-  /* group by register layout; inside a group largest chain first (a greedy
-   * longest-processing-time schedule for the hardware dispatcher) unless
-   * order == 1 (job order) */
+  auto group = [&](uint32_t x) { return b->jobs[x].shape; };
+  /* group by register layout (one launch each: splitting a layout by slab
+   * size gave launches too small to fill the device); inside a group largest
+   * chain first (a greedy longest-processing-time schedule for the hardware
+   * dispatcher) unless order == 1 (job order) */
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    if (b->jobs[x].shape != b->jobs[y].shape) return b->jobs[x].shape < b->jobs[y].shape;
+    if (group(x) != group(y)) return group(x) < group(y);
     return b->cfg.order == 0u ? work(x) > work(y) : false;
   });
   for (uint32_t idx = 0; idx < order.size();) {
-    const uint32_t shape = b->jobs[order[idx]].shape;
+    const uint32_t shape = b->jobs[order[idx]].shape, g = group(order[idx]);
     bbs_batch::Range r{shape, (uint32_t)b->items.size(), 0u, 0u, 0.0};
-    for (; idx < order.size() && b->jobs[order[idx]].shape == shape; idx++) {
+    for (; idx < order.size() && group(order[idx]) == g; idx++) {
       const uint32_t j = order[idx];
-      r.max_nodes = std::max(r.max_nodes, b->jobs[j].n_nodes);
+      r.max_floats = std::max(r.max_floats, job_floats(b->jobs[j]));
       for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
       r.work += work(j) * b->jobs[j].n_chains;
     }
@@ -1131,6 +1145,7 @@ void bbs_batch_config_default(bbs_batch_config *cfg) {
   cfg->persistent = 0u;
   cfg->scoring = BBS_SCORING_AUTO;
   cfg->graph = 0u;
+  cfg->two_wave_layouts = BBS_TWO_WAVE_DEFAULT;
 }
 
 const char *bbs_batch_last_error(const bbs_batch *b) {
@@ -1269,8 +1284,7 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
  * once with the slabs (LDS-bound, slabs per CU) as without (one per SIMD),
  * as the plain kernel's automatic rule decides. count = UINT32_MAX asks for
  * a saturated device (the server). */
-static uint32_t range_slab(uint32_t shape, uint32_t max_nodes, uint32_t count, uint32_t n_cus) {
-  const uint32_t f = bbs::batch_wave_floats(shape, max_nodes);
+static uint32_t range_slab(uint32_t f, uint32_t count, uint32_t n_cus) {
   const uint64_t per_cu = bbs::kMaxGroupLds / (4ull * f);
   const uint64_t cus = n_cus ? n_cus : 1u;
   const uint64_t with_table = std::min<uint64_t>(count, cus * per_cu);
@@ -1281,10 +1295,10 @@ static uint32_t range_slab(uint32_t shape, uint32_t max_nodes, uint32_t count, u
 /* Launch shape of a range: each wave's incremental LDS slab (0: full
  * rescoring) and the waves per workgroup, halved from the configured value
  * until the workgroup's slabs fit in the LDS one workgroup may hold. */
-static void range_launch(uint32_t shape, uint32_t max_nodes, uint32_t wpg_cfg, bool full,
-                         uint32_t count, uint32_t n_cus, uint32_t &wave_floats, uint32_t &wpg) {
+static void range_launch(uint32_t max_floats, uint32_t wpg_cfg, bool full, uint32_t count,
+                         uint32_t n_cus, uint32_t &wave_floats, uint32_t &wpg) {
   wpg = wpg_cfg;
-  wave_floats = full ? 0u : range_slab(shape, max_nodes, count, n_cus);
+  wave_floats = full ? 0u : range_slab(max_floats, count, n_cus);
   while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > bbs::kMaxGroupLds) wpg /= 2u;
   if (4ull * wave_floats * wpg > bbs::kMaxGroupLds) wave_floats = 0u;
 }
@@ -1326,9 +1340,10 @@ static hipError_t enqueue_run(bbs_batch *b) {
     const auto &R = b->ranges[r];
     hipStream_t st = (r % ns) == 0u ? b->stream : b->side[(r % ns) - 1u];
     uint32_t f = 0u, wpg = 1u;
-    range_launch(R.shape, R.max_nodes, b->cfg.waves_per_group,
-                 b->cfg.scoring == BBS_SCORING_FULL, R.count, b->n_cus, f, wpg);
-    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, wpg, f, st);
+    range_launch(R.max_floats, b->cfg.waves_per_group, b->cfg.scoring == BBS_SCORING_FULL,
+                 R.count, b->n_cus, f, wpg);
+    e = bbs::launch_batch_items(a, R.shape, R.first, R.count, wpg, f,
+                                layout_wps(R.shape, b->cfg.two_wave_layouts), st);
   }
   for (uint32_t i = 0; i + 1u < ns && e == hipSuccess; i++) {
     e = hipEventRecord(b->side_done[i], b->side[i]);
@@ -1341,8 +1356,9 @@ int bbs_batch_run(bbs_batch *b, float *makespan_ms) {
   if (!b) return bfail(nullptr, BBS_ERR_ARG, "batch is NULL");
   if (b->cfg.persistent)
     return bfail(b, BBS_ERR_STATE,
-                 "the persistent graph-captured form is switched off: its first GPU "
-                 "test hung and the cause has not been found");
+                 "the persistent kernel form was retired (its first GPU test hung, no "
+                 "cause was found); per-layout launches, optionally replayed as a "
+                 "hipGraph, are the batched form");
   BHIP_TRY(b, hipSetDevice(b->device));
   hipError_t e = hipSuccess;
   if (b->cfg.graph && !b->exec) {
@@ -1423,12 +1439,56 @@ int bbs_batch_get_best(bbs_batch *b, uint32_t job, uint64_t *am, uint64_t *dm,
   return BBS_OK;
 }
 
+/* Best of two runs of a batch (the first warms up); 1e30 if it fails. */
+static int time_batch(const std::vector<bbs_job> &jobs, const bbs_batch_config &c, float &t) {
+  bbs_batch *b = nullptr;
+  int rc = bbs_batch_create(jobs.data(), (uint32_t)jobs.size(), &c, &b);
+  if (rc) return rc;
+  float ms = 0.f;
+  t = 1e30f;
+  for (int r = 0; r < 2 && rc == BBS_OK; r++) {
+    rc = bbs_batch_run(b, &ms);
+    if (r > 0 && ms < t) t = ms;
+  }
+  bbs_batch_destroy(b);
+  return rc;
+}
+
 int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
                        bbs_batch_config *best, float *best_ms) {
   if (!jobs || !best) return bfail(nullptr, BBS_ERR_ARG, "NULL argument");
   std::vector<bbs_job> probe(jobs, jobs + n_jobs);
   for (auto &j : probe)
     if (j.params.n_steps > probe_steps) j.params.n_steps = probe_steps;
+  /* 1. each layout's register budget, its jobs alone on one stream */
+  std::map<uint32_t, std::vector<bbs_job>> by_layout;
+  for (const auto &j : probe) {
+    bbs::BatchJob J;
+    std::string why;
+    const int rc = fill_job(j, J, why);
+    if (rc) return bfail(nullptr, rc, "job rejected: %s", why.c_str());
+    by_layout[J.shape].push_back(j);
+  }
+  bbs_batch_config base;
+  bbs_batch_config_default(&base);
+  base.n_streams = 1u;
+  base.waves_per_group = 1u;
+  uint32_t mask = 0u;
+  for (const auto &kv : by_layout) {
+    const uint32_t shape = kv.first;
+    const uint32_t bit = BBS_LAYOUT_BIT(shape / 100u, shape / 10u % 10u, shape % 10u);
+    float t1 = 0.f, t2 = 0.f;
+    bbs_batch_config c = base;
+    c.two_wave_layouts = 0u;
+    int rc = time_batch(kv.second, c, t1);
+    if (!rc) {
+      c.two_wave_layouts = bit;
+      rc = time_batch(kv.second, c, t2);
+    }
+    if (rc) return rc;
+    if (t2 < t1) mask |= bit;
+  }
+  /* 2. waves per workgroup x streams x item order with those budgets */
   const uint32_t wpgs[3] = {1u, 2u, 4u};
   const uint32_t streams[3] = {1u, 4u, 16u};
   float best_t = 1e30f;
@@ -1436,16 +1496,13 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
   for (uint32_t order = 0; order < 2; order++) {
     for (uint32_t wpg : wpgs) {
       for (uint32_t nst : streams) {
-        bbs_batch_config c{wpg, nst, order, 0u};
-        bbs_batch *b = nullptr;
-        int rc = bbs_batch_create(probe.data(), n_jobs, &c, &b);
-        if (rc) return rc;
-        float ms = 0.f, t = 1e30f;
-        for (int r = 0; r < 2 && rc == BBS_OK; r++) {
-          rc = bbs_batch_run(b, &ms);
-          if (r > 0 && ms < t) t = ms; /* the first run warms up */
-        }
-        bbs_batch_destroy(b);
+        bbs_batch_config c = base;
+        c.waves_per_group = wpg;
+        c.n_streams = nst;
+        c.order = order;
+        c.two_wave_layouts = mask;
+        float t = 0.f;
+        const int rc = time_batch(probe, c, t);
         if (rc) return rc;
         if (t < best_t) {
           best_t = t;
@@ -1997,16 +2054,18 @@ int srv_lane_launch(bbs_server *s, Lane *l) {
   a.best_beta = (float *)s->best_beta.p;
   a.best_score = (float *)s->best_score.p;
   a.best_step = (uint32_t *)s->best_step.p;
-  uint32_t max_nodes = 0u;
-  for (const auto &J : l->h_jobs) max_nodes = std::max(max_nodes, J.n_nodes);
+  uint32_t max_floats = 0u;
+  for (const auto &J : l->h_jobs) max_floats = std::max(max_floats, job_floats(J));
   /* table or full scoring for this slice from its own item count: a lane
    * with few chains (a long job in the tail) takes the faster table even
    * where a saturated device would not; either gives identical chains */
   uint32_t fl = 0u, wpg_r = 1u;
-  range_launch(l->shape, max_nodes, s->cfg.waves_per_group, false, a.n_items, s->n_cus, fl,
+  range_launch(max_floats, s->cfg.waves_per_group, false, a.n_items, s->n_cus, fl,
                wpg_r);
   SHIP_TRY(s, hipEventRecord(l->ev_start, l->stream));
-  SHIP_TRY(s, bbs::launch_batch_items(a, l->shape, 0u, a.n_items, wpg_r, fl, l->stream));
+  SHIP_TRY(s, bbs::launch_batch_items(a, l->shape, 0u, a.n_items, wpg_r, fl,
+                                       layout_wps(l->shape, s->cfg.two_wave_layouts),
+                                       l->stream));
   SHIP_TRY(s, hipEventRecord(l->ev_end, l->stream));
   l->busy = true;
   if (s->n_busy++ == 0u) s->busy_since = srv_now(s);
@@ -2063,6 +2122,7 @@ void bbs_server_config_default(bbs_server_config *cfg) {
   cfg->max_resident_chains = 0u;
   cfg->waves_per_group = 1u;
   cfg->n_streams = 4u;
+  cfg->two_wave_layouts = BBS_TWO_WAVE_DEFAULT;
 }
 
 const char *bbs_server_last_error(const bbs_server *s) {
@@ -2204,7 +2264,7 @@ int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id) {
   j->n_genes = G;
   j->n_steps = job->params.n_steps;
   j->cost = step_cost(j->J);
-  const uint32_t f = range_slab(j->J.shape, j->J.n_nodes, UINT32_MAX, s->n_cus);
+  const uint32_t f = range_slab(job_floats(j->J), UINT32_MAX, s->n_cus);
   j->lds = f ? 4ull * f : 2560ull;
   j->t_submit = srv_now(s);
   j->id = s->next_id++;

@@ -245,14 +245,16 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
     plain_steps = sum(p.n_chains * p.n_steps for p, *_ in jobs_plain)
     cfg, probe_ms = B.batch_autotune(jobs_plain, probe_steps)
     log(f"[bench] batch autotune: wpg={cfg.waves_per_group} streams={cfg.n_streams} "
-        f"order={cfg.order} ({probe_ms:.1f} ms at {probe_steps} steps)")
+        f"order={cfg.order} two-wave layouts {two_wave_codes(cfg.two_wave_layouts)} "
+        f"({probe_ms:.1f} ms at {probe_steps} steps)")
     with B.Batch(jobs_plain, cfg) as b:
         b.run()
         same = all(np.float32(b.best(j)[3]).tobytes() == np.float32(bests_plain[j]).tobytes()
                    for j in range(len(jobs_plain)))
         eager_short = min(b.run() for _ in range(3))
     # the same short batch captured once in a hipGraph and replayed
-    gcfg = B.batch_config(cfg.waves_per_group, cfg.n_streams, cfg.order, graph=True)
+    gcfg = B.batch_config(cfg.waves_per_group, cfg.n_streams, cfg.order, graph=True,
+                          two_wave_layouts=cfg.two_wave_layouts)
     with B.Batch(jobs_plain, gcfg) as g:
         g.run()
         graph_short = min(g.run() for _ in range(3))
@@ -305,7 +307,8 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
                       "graph_replay_ms": graph_short,
                       "graph_best_ll_identical_to_plain": same_graph},
         "autotuned": {"waves_per_group": cfg.waves_per_group,
-                      "n_streams": cfg.n_streams, "order": cfg.order},
+                      "n_streams": cfg.n_streams, "order": cfg.order,
+                      "two_wave_layouts": two_wave_codes(cfg.two_wave_layouts)},
     }
 
 
@@ -313,13 +316,15 @@ def config5_jobs(n_jobs, seed=5):
     return WL.served_stream(n_jobs, seed)
 
 
-def serving_bench(n_jobs):
+def serving_bench(n_jobs, latencies_path=None):
     """Continuous batching on the first n_jobs of the served stream, in real
     time: each job is submitted at its arrival time; latency = completion -
     arrival."""
     jobs = config5_jobs(n_jobs)
+    log(f"[bench] serving {n_jobs} jobs")
     lat = []
     arrival = {}
+    records = []
     with B.Server(B.server_config(hbm_budget_bytes=4 << 30, n_streams=16)) as srv:
         t0 = srv.now()
         nxt = 0
@@ -333,6 +338,8 @@ def serving_bench(n_jobs):
             res, pend, ready = srv.step()
             for r in srv.poll():
                 lat.append(r.done_s - t0 - arrival[r.id])
+                records.append((r.id, arrival[r.id], r.admit_s - t0, r.done_s - t0, r.n_genes,
+                                r.n_cells, r.n_chains, r.n_steps))
                 done += 1
                 if done % 250 == 0:
                     log(f"[bench] served {done}/{n_jobs} jobs at {srv.now() - t0:.1f} s "
@@ -344,6 +351,12 @@ def serving_bench(n_jobs):
     span = jobs[-1][0]
     offered = sum(p.n_chains * p.n_steps for _, (p, *_r) in jobs)
     lat = np.array(lat)
+    if latencies_path:
+        with open(latencies_path, "w") as f:
+            f.write("job,arrival_s,admit_s,done_s,genes,cells,chains,steps\n")
+            for rec in sorted(records):
+                f.write(",".join(f"{x:.6f}" if isinstance(x, float) else str(x) for x in rec)
+                        + "\n")
     return {
         "workload": f"first {n_jobs} jobs of the served stream (Poisson 50 jobs/s, seed 5; "
                     "genes U[16,127], cells U[16,128], chains U[8,256], steps log-uniform "
@@ -361,6 +374,38 @@ def serving_bench(n_jobs):
         "makespan_s": end,
         "slices": rounds,
     }
+
+
+def serve_vs_batch(n_jobs, cap, round_ms=None):
+    """The first n_jobs of the served stream with steps capped at `cap`, all
+    present at once: as one batch (each layout one launch, 16 streams) and
+    through the server (submitted together), to separate the serving loop's
+    own cost from the kernels'."""
+    jobs = [j for _, j in WL.served_stream(n_jobs, max_steps=cap)]
+    total = sum(p.n_chains * p.n_steps for p, *_ in jobs)
+    cfg = B.batch_config(waves_per_group=1, n_streams=16)
+    with B.Batch(jobs, cfg) as b:
+        ms = b.run()
+    out = {"jobs": n_jobs, "steps_cap": cap, "chain_steps": total,
+           "batch_steps_per_s": total / (ms / 1e3)}
+    kw = {"hbm_budget_bytes": 4 << 30, "n_streams": 16}
+    if round_ms is not None:
+        kw["round_ms"] = round_ms
+    with B.Server(B.server_config(**kw)) as srv:
+        t0 = srv.now()
+        for j in jobs:
+            srv.submit(*j)
+        done = 0
+        while done < n_jobs:
+            srv.step()
+            done += len(srv.poll())
+        end = srv.now() - t0
+        steps, busy, slices = srv.stats()
+    out.update({"served_steps_per_s": steps / end, "served_slices": slices,
+                "round_ms": round_ms})
+    log(f"[bench] {n_jobs} jobs capped at {cap} steps: batch {out['batch_steps_per_s']:.3e}, "
+        f"served {out['served_steps_per_s']:.3e} steps/s ({slices} slices)")
+    return out
 
 
 def oracle_check(p, mut, kno, steps, snap):
@@ -416,6 +461,83 @@ def plain_64x63_bench(with_cpu):
     return out
 
 
+def layout_code(genes, cells):
+    """Register layout (node slots, cell slots, gene words) as the library's
+    shape_of picks it, as the code NS*100 + CS*10 + W."""
+    ns = 1 if genes + 1 <= 64 else 2
+    cs = 1 if cells <= 64 else (3 if cells <= 96 else 2)
+    w = max(1, (genes + 31) // 32)
+    if ns == 2:
+        w = max(w, 2)
+    return ns * 100 + cs * 10 + w
+
+
+def two_wave_codes(mask):
+    """Layout codes (NS*100 + CS*10 + W) whose bit is set in a
+    two_wave_layouts mask."""
+    return [ns * 100 + cs * 10 + w for ns in (1, 2) for cs in (1, 2, 3) for w in (1, 2, 3, 4)
+            if mask & B.layout_bit(ns, cs, w)]
+
+
+def layout_profile(steps, only=None, wpg=1, reps=2, served=0):
+    """The 256-dataset mix split by register layout: each layout's datasets
+    (all 64 chains each) as one batch on one stream, timed alone with either
+    register budget (one or two waves per SIMD), then the whole mix with each
+    layout's faster budget. Per layout: items, kernel time, chain steps/s and
+    the reference's int work N C (4W + 2) per step as a fraction of the
+    nominal int32 VALU peak. served > 0: the first `served` jobs of the
+    served stream instead, steps capped at `steps`."""
+    jobs = ([j for _, j in WL.served_stream(served, max_steps=steps)] if served
+            else WL.batch_mix(steps))
+    groups = {}
+    for j, (p, mut, kno, Gj) in enumerate(jobs):
+        groups.setdefault(layout_code(Gj, mut.shape[0]), []).append(j)
+    rows = []
+    mask = 0
+    for code in sorted(groups):
+        if only is not None and code != only:
+            continue
+        sub = [jobs[j] for j in groups[code]]
+        bit = B.layout_bit(code // 100, code // 10 % 10, code % 10)
+        ops = sum(p.n_chains * p.n_steps * (Gj + 1) * int(mut.shape[0]) * (4 * ((Gj + 31) // 32) + 2)
+                  for p, mut, kno, Gj in sub)
+        row = {"layout": code, "datasets": len(sub), "items": sum(p.n_chains for p, *_ in sub),
+               "mean_nodes": float(np.mean([Gj + 1 for *_, Gj in sub])),
+               "mean_cells": float(np.mean([m.shape[0] for _, m, _, _ in sub])),
+               "share_of_alg_work": None}
+        for key, m, sc in (("wps1", 0, "auto"), ("wps2", bit, "auto"), ("full", 0, "full")):
+            cfg = B.batch_config(waves_per_group=wpg, n_streams=1, two_wave_layouts=m,
+                                 scoring=sc)
+            with B.Batch(sub, cfg) as b:
+                ms = min(b.run() for _ in range(reps))
+                total = b.total_steps
+            row[key] = {"ms": ms, "steps_per_s": total / (ms / 1e3),
+                        "alg_int_tops": ops / (ms / 1e3) / 1e12,
+                        "frac_nominal": ops / (ms / 1e3) / 1e12 / P_INT32_TOPS}
+            log(f"[layout] {code} ({key}): {row['datasets']} datasets, "
+                f"{ms:.1f} ms, {total / (ms / 1e3):.3e} steps/s, "
+                f"frac {row[key]['frac_nominal']:.3f}")
+        if row["wps2"]["ms"] < row["wps1"]["ms"]:
+            mask |= bit
+        rows.append((row, ops))
+    all_ops = sum(o for _, o in rows)
+    for row, o in rows:
+        row["share_of_alg_work"] = o / all_ops
+    out = {"steps": steps, "waves_per_group": wpg, "layouts": [r for r, _ in rows],
+           "faster_two_wave_layouts": two_wave_codes(mask)}
+    if only is None:
+        cfg = B.batch_config(waves_per_group=wpg, n_streams=1, two_wave_layouts=mask)
+        with B.Batch(jobs, cfg) as b:
+            ms = min(b.run() for _ in range(reps))
+            total = b.total_steps
+        out["whole_mix"] = {"ms": ms, "steps_per_s": total / (ms / 1e3),
+                            "alg_int_tops": all_ops / (ms / 1e3) / 1e12,
+                            "frac_nominal": all_ops / (ms / 1e3) / 1e12 / P_INT32_TOPS}
+        log(f"[layout] whole mix, one stream: {ms:.1f} ms, "
+            f"frac {out['whole_mix']['frac_nominal']:.3f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +552,29 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200_000)
     ap.add_argument("--serve-jobs", type=int, default=200,
                     help="jobs of the served stream to run (0: skip serving)")
+    ap.add_argument("--layout-profile", action="store_true",
+                    help="only time the 256-dataset mix layout by layout (one JSON line)")
+    ap.add_argument("--layout", type=int, default=None,
+                    help="with --layout-profile: only this layout code (NS*100+CS*10+W)")
+    ap.add_argument("--layout-steps", type=int, default=20_000)
+    ap.add_argument("--layout-served", type=int, default=0, metavar="N",
+                    help="with --layout-profile: the first N served jobs instead of the "
+                         "256-dataset mix")
+    ap.add_argument("--layout-wpg", type=int, default=1,
+                    help="with --layout-profile: waves per workgroup")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the served stream (--serve-jobs jobs), one JSON line")
+    ap.add_argument("--serve-latencies", default=None, metavar="PATH",
+                    help="with the served stream: write each job's arrival, admission and "
+                         "completion times (s) and size to PATH as CSV")
+    ap.add_argument("--serve-vs-batch", type=int, default=0, metavar="N",
+                    help="only compare the first N served jobs (steps capped at "
+                         "--serve-cap) run as one batch and through the server")
+    ap.add_argument("--serve-cap", type=int, default=20_000)
+    ap.add_argument("--round-ms", type=float, default=None,
+                    help="server slice length (default: the library's)")
+    ap.add_argument("--valu-peak-only", action="store_true",
+                    help="only run the int32 VALU probe (for its counters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -443,6 +588,23 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         log(f"[bench] copied {a.copy_calibration} MiB")
+        return
+    if a.valu_peak_only:
+        m0, _, _ = B.generate_dataset(1, 4, 4, 1e-4, 0.2, 0.1)
+        mu0, kn0 = B.encode_matrix(m0)
+        with B.Context(B.default_params(n_chains=1, n_steps=1), mu0, kn0, 4) as c0:
+            ops, insts = c0.measure_valu_peak()
+        print(json.dumps({"int32_ops_per_s": ops, "valu_wave_insts_per_s": insts}), flush=True)
+        return
+    if a.serve_vs_batch:
+        print(json.dumps(serve_vs_batch(a.serve_vs_batch, a.serve_cap, a.round_ms)), flush=True)
+        return
+    if a.serve_only:
+        print(json.dumps(serving_bench(a.serve_jobs, a.serve_latencies)), flush=True)
+        return
+    if a.layout_profile:
+        print(json.dumps(layout_profile(a.layout_steps, a.layout, wpg=a.layout_wpg,
+                                        served=a.layout_served)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "

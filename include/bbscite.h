@@ -251,15 +251,31 @@ typedef struct bbs_batch_config {
                                many hardware queues as GPU_MAX_HW_QUEUES
                                gives the process)                       */
   uint32_t order;           /* 0: largest work first; 1: job order      */
-  uint32_t persistent;      /* 1: persistent graph-captured kernel;
-                               switched off, bbs_batch_run rejects it   */
+  uint32_t persistent;      /* reserved, 0: the persistent kernel form
+                               was retired; bbs_batch_run rejects 1     */
   uint32_t scoring;         /* BBS_SCORING_AUTO (incremental where the
                                pair-score tables fit) or _FULL          */
   uint32_t graph;           /* 1: the first bbs_batch_run captures its
                                launches (initial trees, resets, every
                                layout on every stream) in a hipGraph;
                                later runs replay it                     */
+  uint32_t two_wave_layouts; /* register budget of the incremental scorer
+                               per layout: bit BBS_LAYOUT_BIT(NS, CS, W)
+                               set runs that layout (NS node slots, CS
+                               cell slots, W gene words) with a build
+                               held to 256 registers, so two waves share
+                               a SIMD; clear, with one wave per SIMD and
+                               every register. Results are identical.   */
 } bbs_batch_config;
+
+/* Register layouts (bbs_batch_config.two_wave_layouts): NS = 1 node slot
+ * for up to 64 nodes, else 2; CS = 1 cell slot for up to 64 cells, 3 (a
+ * paired half slot) for up to 96, else 2; W = ceil(genes / 32) 32-bit gene
+ * words (at least 2 with two node slots). */
+#define BBS_LAYOUT_BIT(NS, CS, W) (1u << (((NS)-1u) * 12u + ((CS)-1u) * 4u + ((W)-1u)))
+/* The default: the one-node-slot layouts, which fit 256 registers with at
+ * most a few spills. */
+#define BBS_TWO_WAVE_DEFAULT 0x00000fffu
 
 typedef struct bbs_batch bbs_batch;
 
@@ -278,8 +294,10 @@ int bbs_batch_get_chain_results(bbs_batch *b, uint32_t job, float *best_score,
                                 float *final_beta);
 const char *bbs_batch_last_error(const bbs_batch *b);
 void bbs_batch_destroy(bbs_batch *b);
-/* Times every candidate launch shape on the jobs with n_steps capped at
- * probe_steps and returns the fastest. */
+/* Tunes the launch on the jobs with n_steps capped at probe_steps: first
+ * each layout's register budget (its jobs alone, one wave per SIMD against
+ * two), then waves per workgroup x streams x item order with those budgets;
+ * returns the fastest configuration. */
 int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
                        bbs_batch_config *best, float *best_ms);
 
@@ -313,6 +331,7 @@ typedef struct bbs_server_config {
                                    over these streams; they run at once
                                    only on as many hardware queues as HIP
                                    gives the process (GPU_MAX_HW_QUEUES)   */
+  uint32_t two_wave_layouts;    /* as bbs_batch_config.two_wave_layouts     */
 } bbs_server_config;
 
 typedef struct bbs_job_result {

@@ -455,9 +455,14 @@ def test_batched_items_configs_agree_and_autotune():
     say("[batched configs]")
     jobs = _mix(6, 77, 60)
     ref = None
-    for wpg, nst, order, sc in ((1, 1, 0, "auto"), (4, 2, 1, "auto"), (2, 4, 0, "full"),
-                                (4, 1, 0, "full"), (2, 16, 0, "auto")):
-        with B.Batch(jobs, B.batch_config(wpg, nst, order, scoring=sc)) as b:
+    # the last two: every layout with the two-waves-per-SIMD register budget,
+    # and none
+    for wpg, nst, order, sc, tw in ((1, 1, 0, "auto", None), (4, 2, 1, "auto", None),
+                                    (2, 4, 0, "full", None), (4, 1, 0, "full", None),
+                                    (2, 16, 0, "auto", None), (4, 4, 0, "auto", 0xFFFFFF),
+                                    (1, 4, 1, "auto", 0)):
+        with B.Batch(jobs, B.batch_config(wpg, nst, order, scoring=sc,
+                                          two_wave_layouts=tw)) as b:
             b.run()
             res = [b.chain_results(j)[2] for j in range(len(jobs))]
         if ref is None:
@@ -467,6 +472,7 @@ def test_batched_items_configs_agree_and_autotune():
                 assert np.array_equal(_bits(x), _bits(y))
     cfg, ms = B.batch_autotune(jobs, probe_steps=20)
     assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 4, 16) and ms > 0
+    assert cfg.two_wave_layouts & ~0xFFFFFF == 0
 
 
 def test_batched_graph_replays_equal_eager():
