@@ -114,6 +114,10 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
                               uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
                               uint32_t waves_per_simd, hipStream_t stream);
+/* Chain waves of one launch that a CU holds at once (registers and LDS of
+ * the kernel instance that launch_batch_items would pick). */
+hipError_t batch_items_occupancy(uint32_t shape, uint32_t waves_per_group, uint32_t wave_floats,
+                                 uint32_t waves_per_simd, uint32_t *waves_per_cu);
 /* LDS floats of an incremental chain wave for datasets of up to max_nodes
  * nodes and max_cells cells in this layout */
 uint32_t batch_wave_floats(uint32_t shape, uint32_t max_nodes, uint32_t max_cells);

@@ -1222,6 +1222,33 @@ static hipError_t launch_items_shape(const BatchArgs &a, uint32_t first, uint32_
   return hipGetLastError();
 }
 
+template <int NS, int CS, int W>
+static hipError_t items_occupancy_shape(uint32_t wpg, uint32_t wave_floats, uint32_t wps,
+                                        int *groups) {
+  const size_t lds = 4ull * wave_floats * wpg;
+  if (!wave_floats)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        groups, batch_items_kernel<NS, CS, W, false, 1>, (int)(64u * wpg), 0);
+  if (wps == 2u)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        groups, batch_items_kernel<NS, CS, W, true, 2>, (int)(64u * wpg), lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      groups, batch_items_kernel<NS, CS, W, true, 1>, (int)(64u * wpg), lds);
+}
+
+hipError_t batch_items_occupancy(uint32_t shape, uint32_t waves_per_group, uint32_t wave_floats,
+                                 uint32_t waves_per_simd, uint32_t *waves_per_cu) {
+  int groups = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define XO(NS_, CS_, W_)                                                       \
+  if (shape == (uint32_t)BBS_SHAPE_CODE(NS_, CS_, W_))                        \
+    e = items_occupancy_shape<NS_, CS_, W_>(waves_per_group, wave_floats, waves_per_simd, &groups);
+  BBS_SHAPES(XO)
+#undef XO
+  *waves_per_cu = e == hipSuccess ? (uint32_t)groups * waves_per_group : 0u;
+  return e;
+}
+
 hipError_t launch_batch_items(const BatchArgs &a, uint32_t shape, uint32_t first,
                               uint32_t count, uint32_t waves_per_group, uint32_t wave_floats,
                               uint32_t waves_per_simd, hipStream_t stream) {

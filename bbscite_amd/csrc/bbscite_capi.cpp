@@ -1079,6 +1079,85 @@ uint32_t job_floats(const bbs::BatchJob &J) {
   return bbs::batch_wave_floats(J.shape, J.n_nodes, J.n_cells);
 }
 
+/* Incremental slab of a layout (floats per wave), or 0 where the table
+ * would cost concurrency: a range of `count` items runs on as many waves at
+ * once with the slabs (LDS-bound, slabs per CU) as without (one per SIMD),
+ * as the plain kernel's automatic rule decides. count = UINT32_MAX asks for
+ * a saturated device (the server). */
+uint32_t range_slab(uint32_t f, uint32_t count, uint32_t n_cus) {
+  const uint64_t per_cu = bbs::kMaxGroupLds / (4ull * f);
+  const uint64_t cus = n_cus ? n_cus : 1u;
+  const uint64_t with_table = std::min<uint64_t>(count, cus * per_cu);
+  const uint64_t without = std::min<uint64_t>(count, cus * 4u);
+  return per_cu > 0u && with_table >= without ? f : 0u;
+}
+
+/* Launch shape of a range: each wave's incremental LDS slab (0: full
+ * rescoring) and the waves per workgroup, halved from the configured value
+ * until the workgroup's slabs fit in the LDS one workgroup may hold. */
+void range_launch(uint32_t max_floats, uint32_t wpg_cfg, bool full, uint32_t count,
+                  uint32_t n_cus, uint32_t &wave_floats, uint32_t &wpg) {
+  wpg = wpg_cfg;
+  wave_floats = full ? 0u : range_slab(max_floats, count, n_cus);
+  while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > bbs::kMaxGroupLds) wpg /= 2u;
+  if (4ull * wave_floats * wpg > bbs::kMaxGroupLds) wave_floats = 0u;
+}
+
+/* Chain waves per CU of a launch of `count` items of a layout whose largest
+ * slab is max_floats, in the launch shape range_launch picks (0 if the
+ * occupancy cannot be queried). */
+uint32_t launch_waves_per_cu(uint32_t shape, uint32_t max_floats, uint32_t count,
+                             uint32_t wpg_cfg, bool full, uint32_t n_cus,
+                             uint32_t two_wave_layouts) {
+  uint32_t f = 0u, wpg = 1u, w = 0u;
+  range_launch(max_floats, wpg_cfg, full, count, n_cus, f, wpg);
+  if (bbs::batch_items_occupancy(shape, wpg, f, layout_wps(shape, two_wave_layouts), &w) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return 0u;
+  }
+  return w;
+}
+
+/* Splits the jobs of one layout (`items` chains in all) into the class whose
+ * own slabs let a CU hold the most chain waves and the rest: small[k] = 1 for
+ * the first. Returns false (no split) when every job is in the first class,
+ * when it has fewer chains than the device has CUs, or with full scoring. */
+bool slab_classes(uint32_t shape, const std::vector<const bbs::BatchJob *> &js, uint32_t items,
+                  uint32_t wpg_cfg, bool full, uint32_t n_cus, uint32_t two_wave_layouts,
+                  std::vector<char> &small) {
+  small.assign(js.size(), 1);
+  if (full || js.size() < 2u) return false;
+  std::vector<uint32_t> occ(js.size());
+  uint32_t top = 0u;
+  for (size_t k = 0; k < js.size(); k++) {
+    occ[k] = launch_waves_per_cu(shape, job_floats(*js[k]), items, wpg_cfg, false, n_cus,
+                                 two_wave_layouts);
+    top = std::max(top, occ[k]);
+  }
+  uint32_t small_items = 0u;
+  bool any_rest = false;
+  for (size_t k = 0; k < js.size(); k++) {
+    small[k] = occ[k] == top ? 1 : 0;
+    if (small[k])
+      small_items += js[k]->n_chains;
+    else
+      any_rest = true;
+  }
+  if (!any_rest || small_items < n_cus) {
+    small.assign(js.size(), 1);
+    return false;
+  }
+  return true;
+}
+
+/* Items grouped by register layout, one launch (range) per layout and at
+ * most two per layout: when some datasets' own slabs let a CU hold more
+ * chain waves than the layout's largest slab does, those datasets (at least
+ * one item per CU of them) get a range of their own, with a smaller slab.
+ * Inside a range the largest chain comes first (a greedy longest-
+ * processing-time schedule for the hardware dispatcher) unless order == 1
+ * (job order). Ranges are sorted heaviest first. */
 void order_items(bbs_batch *b) {
   b->items.clear();
   b->ranges.clear();
@@ -1088,26 +1167,38 @@ void order_items(bbs_batch *b) {
     const auto &A = b->jobs[x];
     return step_cost(A) * (A.step_end - A.step_begin);
   };
-  auto group = [&](uint32_t x) { return b->jobs[x].shape; };
-  /* group by register layout (one launch each: splitting a layout by slab
-   * size gave launches too small to fill the device); inside a group largest
-   * chain first (a greedy longest-processing-time schedule for the hardware
-   * dispatcher) unless order == 1 (job order) */
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    if (group(x) != group(y)) return group(x) < group(y);
+    if (b->jobs[x].shape != b->jobs[y].shape) return b->jobs[x].shape < b->jobs[y].shape;
     return b->cfg.order == 0u ? work(x) > work(y) : false;
   });
-  for (uint32_t idx = 0; idx < order.size();) {
-    const uint32_t shape = b->jobs[order[idx]].shape, g = group(order[idx]);
-    bbs_batch::Range r{shape, (uint32_t)b->items.size(), 0u, 0u, 0.0};
-    for (; idx < order.size() && group(order[idx]) == g; idx++) {
-      const uint32_t j = order[idx];
+  auto emit = [&](const std::vector<uint32_t> &js) {
+    if (js.empty()) return;
+    bbs_batch::Range r{b->jobs[js[0]].shape, (uint32_t)b->items.size(), 0u, 0u, 0.0};
+    for (uint32_t j : js) {
       r.max_floats = std::max(r.max_floats, job_floats(b->jobs[j]));
       for (uint32_t c = 0; c < b->jobs[j].n_chains; c++) b->items.push_back((j << 16) | c);
       r.work += work(j) * b->jobs[j].n_chains;
     }
     r.count = (uint32_t)b->items.size() - r.first;
     if (r.count) b->ranges.push_back(r);
+  };
+  for (uint32_t idx = 0; idx < order.size();) {
+    const uint32_t shape = b->jobs[order[idx]].shape;
+    std::vector<uint32_t> group;
+    uint32_t items = 0u;
+    for (; idx < order.size() && b->jobs[order[idx]].shape == shape; idx++) {
+      group.push_back(order[idx]);
+      items += b->jobs[order[idx]].n_chains;
+    }
+    std::vector<const bbs::BatchJob *> js;
+    for (uint32_t j : group) js.push_back(&b->jobs[j]);
+    std::vector<char> cls;
+    slab_classes(shape, js, items, b->cfg.waves_per_group, b->cfg.scoring == BBS_SCORING_FULL,
+                 b->n_cus, b->cfg.two_wave_layouts, cls);
+    std::vector<uint32_t> small, rest;
+    for (size_t k = 0; k < group.size(); k++) (cls[k] ? small : rest).push_back(group[k]);
+    emit(small);
+    emit(rest);
   }
   std::stable_sort(b->ranges.begin(), b->ranges.end(),
                    [](const bbs_batch::Range &x, const bbs_batch::Range &y) {
@@ -1140,7 +1231,7 @@ extern "C" {
 void bbs_batch_config_default(bbs_batch_config *cfg) {
   if (!cfg) return;
   cfg->waves_per_group = 1u;
-  cfg->n_streams = 4u;
+  cfg->n_streams = 16u;
   cfg->order = 0u;
   cfg->persistent = 0u;
   cfg->scoring = BBS_SCORING_AUTO;
@@ -1234,7 +1325,6 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
     rows_total += (uint64_t)J.n_chains * J.n_nodes;
     b->total_steps += (uint64_t)J.n_chains * (J.step_end - J.step_begin);
   }
-  order_items(b);
   hipDeviceProp_t prop;
   hipError_t e = hipSetDevice(dev);
   if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
@@ -1251,6 +1341,7 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
     return bfail(nullptr, BBS_ERR_HIP, "HIP setup failed: %s", hipGetErrorString(e));
   }
   b->n_cus = (uint32_t)prop.multiProcessorCount;
+  order_items(b); /* needs the device current (occupancy queries) */
   const size_t rows_b = rows_total * 16u, sc_b = chains_total * 4u;
   int rc;
   if ((rc = balloc(b, b->d_jobs, b->jobs.size() * sizeof(bbs::BatchJob))) ||
@@ -1277,30 +1368,6 @@ int bbs_batch_create(const bbs_job *jobs, uint32_t n_jobs, const bbs_batch_confi
   }
   *out = b;
   return BBS_OK;
-}
-
-/* Incremental slab of a layout (floats per wave), or 0 where the table
- * would cost concurrency: a range of `count` items runs on as many waves at
- * once with the slabs (LDS-bound, slabs per CU) as without (one per SIMD),
- * as the plain kernel's automatic rule decides. count = UINT32_MAX asks for
- * a saturated device (the server). */
-static uint32_t range_slab(uint32_t f, uint32_t count, uint32_t n_cus) {
-  const uint64_t per_cu = bbs::kMaxGroupLds / (4ull * f);
-  const uint64_t cus = n_cus ? n_cus : 1u;
-  const uint64_t with_table = std::min<uint64_t>(count, cus * per_cu);
-  const uint64_t without = std::min<uint64_t>(count, cus * 4u);
-  return per_cu > 0u && with_table >= without ? f : 0u;
-}
-
-/* Launch shape of a range: each wave's incremental LDS slab (0: full
- * rescoring) and the waves per workgroup, halved from the configured value
- * until the workgroup's slabs fit in the LDS one workgroup may hold. */
-static void range_launch(uint32_t max_floats, uint32_t wpg_cfg, bool full, uint32_t count,
-                         uint32_t n_cus, uint32_t &wave_floats, uint32_t &wpg) {
-  wpg = wpg_cfg;
-  wave_floats = full ? 0u : range_slab(max_floats, count, n_cus);
-  while (wave_floats && wpg > 1u && 4ull * wave_floats * wpg > bbs::kMaxGroupLds) wpg /= 2u;
-  if (4ull * wave_floats * wpg > bbs::kMaxGroupLds) wave_floats = 0u;
 }
 
 /* Everything one run puts on the streams: initial trees and cleared
@@ -1477,6 +1544,14 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
   for (const auto &kv : by_layout) {
     const uint32_t shape = kv.first;
     const uint32_t bit = BBS_LAYOUT_BIT(shape / 100u, shape / 10u % 10u, shape % 10u);
+    /* probe only where the budgets differ: the occupancy the registers
+     * allow (a minimal slab) of the one-wave build against the two-wave
+     * build; where they are equal, the one-wave build (no spills) is kept */
+    uint32_t occ1 = 0u, occ2 = 0u;
+    const bool q1 = bbs::batch_items_occupancy(shape, 1u, 4u, 1u, &occ1) == hipSuccess;
+    const bool q2 = bbs::batch_items_occupancy(shape, 1u, 4u, 2u, &occ2) == hipSuccess;
+    (void)hipGetLastError();
+    if (q1 && q2 && occ2 <= occ1) continue;
     float t1 = 0.f, t2 = 0.f;
     bbs_batch_config c = base;
     c.two_wave_layouts = 0u;
@@ -1488,9 +1563,12 @@ int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_step
     if (rc) return rc;
     if (t2 < t1) mask |= bit;
   }
-  /* 2. waves per workgroup x streams x item order with those budgets */
-  const uint32_t wpgs[3] = {1u, 2u, 4u};
-  const uint32_t streams[3] = {1u, 4u, 16u};
+  /* 2. waves per workgroup x streams x item order with those budgets (LDS
+   * is allocated per workgroup, so one or two waves per group pack CUs
+   * best; 16 streams overlap the layouts on as many hardware queues as the
+   * process has) */
+  const uint32_t wpgs[2] = {1u, 2u};
+  const uint32_t streams[2] = {2u, 16u};
   float best_t = 1e30f;
   bool found = false;
   for (uint32_t order = 0; order < 2; order++) {
@@ -2028,14 +2106,35 @@ int srv_lane_launch(bbs_server *s, Lane *l) {
                                        0xff800000u, J.n_chains, l->stream));
     }
   }
-  /* items: the largest work first */
+  /* two launches when some jobs' own slabs fit more chain waves per CU
+   * than the lane's largest slab (slab_classes): those jobs first; inside
+   * each class the largest work first */
+  uint32_t lane_items = 0u;
+  std::vector<const bbs::BatchJob *> js(n);
+  for (uint32_t r = 0; r < n; r++) {
+    js[r] = &l->h_jobs[r];
+    lane_items += l->h_jobs[r].n_chains;
+  }
+  std::vector<char> cls;
+  const bool split = slab_classes(l->shape, js, lane_items, s->cfg.waves_per_group, false,
+                                  s->n_cus, s->cfg.two_wave_layouts, cls);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    if (cls[x] != cls[y]) return cls[x] > cls[y];
     const double wx = l->jobs[x]->cost * (double)(l->h_jobs[x].step_end - l->h_jobs[x].step_begin);
     const double wy = l->jobs[y]->cost * (double)(l->h_jobs[y].step_end - l->h_jobs[y].step_begin);
     return wx > wy;
   });
-  for (uint32_t r : order)
+  uint32_t n_first = 0u;
+  uint32_t floats_first = 0u, floats_second = 0u;
+  for (uint32_t r : order) {
     for (uint32_t c = 0; c < l->h_jobs[r].n_chains; c++) l->h_items.push_back((r << 16) | c);
+    if (cls[r]) {
+      n_first += l->h_jobs[r].n_chains;
+      floats_first = std::max(floats_first, job_floats(l->h_jobs[r]));
+    } else {
+      floats_second = std::max(floats_second, job_floats(l->h_jobs[r]));
+    }
+  }
   SHIP_TRY(s, hipMemcpyAsync(l->d_jobs.p, l->h_jobs.data(), n * sizeof(bbs::BatchJob),
                              hipMemcpyHostToDevice, l->stream));
   SHIP_TRY(s, hipMemcpyAsync(l->d_items.p, l->h_items.data(), l->h_items.size() * 4u,
@@ -2054,18 +2153,22 @@ int srv_lane_launch(bbs_server *s, Lane *l) {
   a.best_beta = (float *)s->best_beta.p;
   a.best_score = (float *)s->best_score.p;
   a.best_step = (uint32_t *)s->best_step.p;
-  uint32_t max_floats = 0u;
-  for (const auto &J : l->h_jobs) max_floats = std::max(max_floats, job_floats(J));
-  /* table or full scoring for this slice from its own item count: a lane
+  /* table or full scoring for each launch from its own item count: a lane
    * with few chains (a long job in the tail) takes the faster table even
-   * where a saturated device would not; either gives identical chains */
-  uint32_t fl = 0u, wpg_r = 1u;
-  range_launch(max_floats, s->cfg.waves_per_group, false, a.n_items, s->n_cus, fl,
-               wpg_r);
+   * where a saturated device would not; either gives identical chains. The
+   * two launches of a split slice follow each other on the lane's stream
+   * (other lanes keep the device busy meanwhile). */
+  const uint32_t wps = layout_wps(l->shape, s->cfg.two_wave_layouts);
+  auto launch = [&](uint32_t first, uint32_t count, uint32_t max_floats) -> int {
+    uint32_t fl = 0u, wpg_r = 1u;
+    range_launch(max_floats, s->cfg.waves_per_group, false, count, s->n_cus, fl, wpg_r);
+    SHIP_TRY(s, bbs::launch_batch_items(a, l->shape, first, count, wpg_r, fl, wps, l->stream));
+    return BBS_OK;
+  };
   SHIP_TRY(s, hipEventRecord(l->ev_start, l->stream));
-  SHIP_TRY(s, bbs::launch_batch_items(a, l->shape, 0u, a.n_items, wpg_r, fl,
-                                       layout_wps(l->shape, s->cfg.two_wave_layouts),
-                                       l->stream));
+  int rc = split ? launch(0u, n_first, floats_first) : launch(0u, a.n_items, floats_first);
+  if (!rc && split) rc = launch(n_first, a.n_items - n_first, floats_second);
+  if (rc) return rc;
   SHIP_TRY(s, hipEventRecord(l->ev_end, l->stream));
   l->busy = true;
   if (s->n_busy++ == 0u) s->busy_since = srv_now(s);

@@ -30,10 +30,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The server runs one lane (stream) per register layout, on up to 16
+# The batched form and the server spread their launches over up to 16
 # streams; HIP runs them at once only on as many hardware queues as the
-# process asks for (read when HIP initialises, 4 by default).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# process asks for (read when HIP initialises). The environment may already
+# hold a smaller value (4, HIP's default), so it is set, not defaulted.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 from bbscite_amd import _capi as B  # noqa: E402
 from bbscite_amd import workloads as WL  # noqa: E402
@@ -53,6 +54,9 @@ P_INT8_MFMA_TOPS = 5000.0
 # plain 1 GiB device copy gives those counters: 2,589 KiB x 2 + 6,304 KiB.
 TRAFFIC_100K_WINDOW = (2589 * 2 + 6304) * 1024
 FPGA_KSTEPS = 2938.25  # reference's published chain steps/s at 95 x 94 (kilo)
+# Served slices of about 50 ms: fewer relaunches than the library's 20 ms
+# default; a newly admitted job waits at most one slice for its lane.
+SERVE_ROUND_MS = 50.0
 
 
 def log(*a):
@@ -230,7 +234,7 @@ def config4_jobs(steps, chains=64, n_datasets=256):
     return WL.batch_mix(steps, chains, n_datasets)
 
 
-def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
+def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=20_000):
     """Config 4: the 256-dataset uneven mix, plain launches vs the batched
     per-layout launches. Plain launches run one dataset each, serially. At
     the plain step count the batch is also timed as an eager run and as a
@@ -243,7 +247,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=2_000):
             t_plain += ctx.run() / 1e3
             bests_plain.append(ctx.best()[3])
     plain_steps = sum(p.n_chains * p.n_steps for p, *_ in jobs_plain)
-    cfg, probe_ms = B.batch_autotune(jobs_plain, probe_steps)
+    cfg, probe_ms = B.batch_autotune(config4_jobs(probe_steps), probe_steps)
     log(f"[bench] batch autotune: wpg={cfg.waves_per_group} streams={cfg.n_streams} "
         f"order={cfg.order} two-wave layouts {two_wave_codes(cfg.two_wave_layouts)} "
         f"({probe_ms:.1f} ms at {probe_steps} steps)")
@@ -316,21 +320,28 @@ def config5_jobs(n_jobs, seed=5):
     return WL.served_stream(n_jobs, seed)
 
 
-def serving_bench(n_jobs, latencies_path=None):
+def serving_bench(n_jobs, latencies_path=None, deadline_s=None):
     """Continuous batching on the first n_jobs of the served stream, in real
     time: each job is submitted at its arrival time; latency = completion -
-    arrival."""
+    arrival. With deadline_s the run stops after that many seconds and
+    reports the jobs completed by then (complete: false)."""
     jobs = config5_jobs(n_jobs)
     log(f"[bench] serving {n_jobs} jobs")
     lat = []
     arrival = {}
     records = []
-    with B.Server(B.server_config(hbm_budget_bytes=4 << 30, n_streams=16)) as srv:
+    with B.Server(B.server_config(hbm_budget_bytes=4 << 30, n_streams=16,
+                                  round_ms=SERVE_ROUND_MS)) as srv:
         t0 = srv.now()
         nxt = 0
         done = 0
+        complete = True
         while done < n_jobs:
             now = srv.now() - t0
+            if deadline_s is not None and now > deadline_s:
+                complete = False
+                log(f"[bench] serving deadline {deadline_s:.0f} s reached with {done} jobs done")
+                break
             while nxt < n_jobs and jobs[nxt][0] <= now:
                 jid = srv.submit(*jobs[nxt][1])
                 arrival[jid] = jobs[nxt][0]
@@ -344,7 +355,7 @@ def serving_bench(n_jobs, latencies_path=None):
                 if done % 250 == 0:
                     log(f"[bench] served {done}/{n_jobs} jobs at {srv.now() - t0:.1f} s "
                         f"({res} resident, {pend} pending)")
-            if res == 0 and pend == 0 and nxt < n_jobs:
+            if res == 0 and pend == 0 and nxt < n_jobs and done < n_jobs:
                 time.sleep(max(0.0, jobs[nxt][0] - (srv.now() - t0)))
         end = srv.now() - t0
         chain_steps, busy, rounds = srv.stats()
@@ -362,6 +373,8 @@ def serving_bench(n_jobs, latencies_path=None):
                     "genes U[16,127], cells U[16,128], chains U[8,256], steps log-uniform "
                     "[1e4,1e6])",
         "jobs": n_jobs,
+        "jobs_completed": done,
+        "complete": complete,
         "subset_of_10000_job_stream": n_jobs < 10000,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p90_ms": float(np.percentile(lat, 90) * 1e3),
@@ -388,9 +401,8 @@ def serve_vs_batch(n_jobs, cap, round_ms=None):
         ms = b.run()
     out = {"jobs": n_jobs, "steps_cap": cap, "chain_steps": total,
            "batch_steps_per_s": total / (ms / 1e3)}
-    kw = {"hbm_budget_bytes": 4 << 30, "n_streams": 16}
-    if round_ms is not None:
-        kw["round_ms"] = round_ms
+    kw = {"hbm_budget_bytes": 4 << 30, "n_streams": 16,
+          "round_ms": SERVE_ROUND_MS if round_ms is None else round_ms}
     with B.Server(B.server_config(**kw)) as srv:
         t0 = srv.now()
         for j in jobs:
@@ -479,9 +491,10 @@ def two_wave_codes(mask):
             if mask & B.layout_bit(ns, cs, w)]
 
 
-def layout_profile(steps, only=None, wpg=1, reps=2, served=0):
+def layout_profile(steps, only=None, wpg=1, reps=2, served=0, streams=2):
     """The 256-dataset mix split by register layout: each layout's datasets
-    (all 64 chains each) as one batch on one stream, timed alone with either
+    (all 64 chains each) as one batch (its one or two launches on their own
+    streams), timed alone with either
     register budget (one or two waves per SIMD), then the whole mix with each
     layout's faster budget. Per layout: items, kernel time, chain steps/s and
     the reference's int work N C (4W + 2) per step as a fraction of the
@@ -506,7 +519,7 @@ def layout_profile(steps, only=None, wpg=1, reps=2, served=0):
                "mean_cells": float(np.mean([m.shape[0] for _, m, _, _ in sub])),
                "share_of_alg_work": None}
         for key, m, sc in (("wps1", 0, "auto"), ("wps2", bit, "auto"), ("full", 0, "full")):
-            cfg = B.batch_config(waves_per_group=wpg, n_streams=1, two_wave_layouts=m,
+            cfg = B.batch_config(waves_per_group=wpg, n_streams=streams, two_wave_layouts=m,
                                  scoring=sc)
             with B.Batch(sub, cfg) as b:
                 ms = min(b.run() for _ in range(reps))
@@ -523,17 +536,18 @@ def layout_profile(steps, only=None, wpg=1, reps=2, served=0):
     all_ops = sum(o for _, o in rows)
     for row, o in rows:
         row["share_of_alg_work"] = o / all_ops
-    out = {"steps": steps, "waves_per_group": wpg, "layouts": [r for r, _ in rows],
+    out = {"steps": steps, "waves_per_group": wpg, "streams": streams,
+           "layouts": [r for r, _ in rows],
            "faster_two_wave_layouts": two_wave_codes(mask)}
     if only is None:
-        cfg = B.batch_config(waves_per_group=wpg, n_streams=1, two_wave_layouts=mask)
+        cfg = B.batch_config(waves_per_group=wpg, n_streams=streams, two_wave_layouts=mask)
         with B.Batch(jobs, cfg) as b:
             ms = min(b.run() for _ in range(reps))
             total = b.total_steps
         out["whole_mix"] = {"ms": ms, "steps_per_s": total / (ms / 1e3),
                             "alg_int_tops": all_ops / (ms / 1e3) / 1e12,
                             "frac_nominal": all_ops / (ms / 1e3) / 1e12 / P_INT32_TOPS}
-        log(f"[layout] whole mix, one stream: {ms:.1f} ms, "
+        log(f"[layout] whole mix, {streams} streams: {ms:.1f} ms, "
             f"frac {out['whole_mix']['frac_nominal']:.3f}")
     return out
 
@@ -560,6 +574,7 @@ def main():
     ap.add_argument("--layout-served", type=int, default=0, metavar="N",
                     help="with --layout-profile: the first N served jobs instead of the "
                          "256-dataset mix")
+    ap.add_argument("--layout-streams", type=int, default=2)
     ap.add_argument("--layout-wpg", type=int, default=1,
                     help="with --layout-profile: waves per workgroup")
     ap.add_argument("--serve-only", action="store_true",
@@ -567,6 +582,11 @@ def main():
     ap.add_argument("--serve-latencies", default=None, metavar="PATH",
                     help="with the served stream: write each job's arrival, admission and "
                          "completion times (s) and size to PATH as CSV")
+    ap.add_argument("--batch-shapes", default=None, metavar="WPG:STREAMS,...",
+                    help="only time the 256-dataset batch at --batch-steps steps with each "
+                         "given launch shape (waves per group : streams), no autotuning")
+    ap.add_argument("--serve-deadline", type=float, default=None, metavar="S",
+                    help="with --serve-only: stop after S seconds and report what completed")
     ap.add_argument("--serve-vs-batch", type=int, default=0, metavar="N",
                     help="only compare the first N served jobs (steps capped at "
                          "--serve-cap) run as one batch and through the server")
@@ -596,15 +616,31 @@ def main():
             ops, insts = c0.measure_valu_peak()
         print(json.dumps({"int32_ops_per_s": ops, "valu_wave_insts_per_s": insts}), flush=True)
         return
+    if a.batch_shapes:
+        jobs = config4_jobs(a.batch_steps)
+        res = []
+        for shape in a.batch_shapes.split(","):
+            wpg, nst = (int(x) for x in shape.split(":"))
+            with B.Batch(jobs, B.batch_config(waves_per_group=wpg, n_streams=nst)) as b:
+                ms = b.run()
+                total = b.total_steps
+            res.append({"waves_per_group": wpg, "n_streams": nst, "ms": ms,
+                        "steps_per_s": total / (ms / 1e3)})
+            log(f"[bench] batch wpg={wpg} streams={nst}: {ms:.1f} ms, "
+                f"{total / (ms / 1e3):.3e} steps/s")
+        print(json.dumps({"steps": a.batch_steps, "shapes": res}), flush=True)
+        return
     if a.serve_vs_batch:
         print(json.dumps(serve_vs_batch(a.serve_vs_batch, a.serve_cap, a.round_ms)), flush=True)
         return
     if a.serve_only:
-        print(json.dumps(serving_bench(a.serve_jobs, a.serve_latencies)), flush=True)
+        print(json.dumps(serving_bench(a.serve_jobs, a.serve_latencies, a.serve_deadline)),
+              flush=True)
         return
     if a.layout_profile:
         print(json.dumps(layout_profile(a.layout_steps, a.layout, wpg=a.layout_wpg,
-                                        served=a.layout_served)), flush=True)
+                                        served=a.layout_served, streams=a.layout_streams)),
+              flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "

@@ -295,9 +295,10 @@ int bbs_batch_get_chain_results(bbs_batch *b, uint32_t job, float *best_score,
 const char *bbs_batch_last_error(const bbs_batch *b);
 void bbs_batch_destroy(bbs_batch *b);
 /* Tunes the launch on the jobs with n_steps capped at probe_steps: first
- * each layout's register budget (its jobs alone, one wave per SIMD against
- * two), then waves per workgroup x streams x item order with those budgets;
- * returns the fastest configuration. */
+ * each layout's register budget where the two builds differ in occupancy
+ * (its jobs alone, one wave per SIMD against two), then waves per workgroup
+ * {1, 2} x streams {2, 16} x item order with those budgets; returns the
+ * fastest configuration. */
 int bbs_batch_autotune(const bbs_job *jobs, uint32_t n_jobs, uint32_t probe_steps,
                        bbs_batch_config *best, float *best_ms);
 

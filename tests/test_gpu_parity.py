@@ -471,7 +471,7 @@ def test_batched_items_configs_agree_and_autotune():
             for x, y in zip(ref, res):
                 assert np.array_equal(_bits(x), _bits(y))
     cfg, ms = B.batch_autotune(jobs, probe_steps=20)
-    assert cfg.waves_per_group in (1, 2, 4) and cfg.n_streams in (1, 4, 16) and ms > 0
+    assert cfg.waves_per_group in (1, 2) and cfg.n_streams in (2, 16) and ms > 0
     assert cfg.two_wave_layouts & ~0xFFFFFF == 0
 
 
