@@ -168,6 +168,14 @@ __device__ __forceinline__ float vmax(float a, float b) {
   return r;
 }
 
+/* vmax(vmax(a, b), c) as one v_max3_f32, which gfx950 defines as exactly
+ * that pair of v_max_f32 operations (same NaN and signed-zero rules). */
+__device__ __forceinline__ float vmax3(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 /* Cell slots per lane: CS = 1 holds cells 0..63; CS = 2 cells 0..127 in two
  * slots; CS = 3 cells 0..95 with the second slot "paired": lanes l and
  * l + 32 both hold cell 64 + l % 32 and score different nodes, so the 32
@@ -392,11 +400,16 @@ template <int CS>
 __device__ __forceinline__ float ordered_sum(uint32_t n_cells, WaveLds &S) {
   wave_lds_sync();
   float T;
+  /* padded to the next multiple of 16 cells */
   if constexpr (CS == 1) {
-    T = n_cells <= 16u ? sum_padded<4>(S.sums) : n_cells <= 32u ? sum_padded<8>(S.sums)
-                                                                 : sum_padded<16>(S.sums);
+    T = n_cells <= 16u   ? sum_padded<4>(S.sums)
+        : n_cells <= 32u ? sum_padded<8>(S.sums)
+        : n_cells <= 48u ? sum_padded<12>(S.sums)
+                         : sum_padded<16>(S.sums);
+  } else if constexpr (CS == 3) {
+    T = n_cells <= 80u ? sum_padded<20>(S.sums) : sum_padded<24>(S.sums);
   } else {
-    T = sum_padded<CS == 3 ? 24 : 32>(S.sums);
+    T = n_cells <= 112u ? sum_padded<28>(S.sums) : sum_padded<32>(S.sums);
   }
   wave_lds_sync();
   return rfl_f(T);
@@ -756,7 +769,7 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
         trow[j][p.a] = sa;
         trow[j][p.b] = sb;
       } else {
-        R[j] = vmax(vmax(R[j], sa), sb);
+        R[j] = vmax3(R[j], sa, sb);
       }
     }
     if constexpr (CS == 3) {
@@ -869,10 +882,8 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
     }
   };
   auto mx4 = [&](float &u, const float4 &x, const float4 &q) {
-    u = vmax(u, x.x + q.x);
-    u = vmax(u, x.y + q.y);
-    u = vmax(u, x.z + q.z);
-    u = vmax(u, x.w + q.w);
+    u = vmax3(u, x.x + q.x, x.y + q.y);
+    u = vmax3(u, x.z + q.z, x.w + q.w);
   };
   auto use = [&](const Grp &b) {
 #pragma unroll
@@ -939,8 +950,10 @@ struct PriorArgs {
   float lg_ab, lg_a, lg_b;
 };
 
-__device__ __forceinline__ float logscore_beta(const PriorArgs &p, float beta) {
-  return p.lg_ab + p.am1 * bbs_logf(beta) + p.bm1 * bbs_logf(1.0f - beta) -
+/* kb: the logf coefficients' index base (bbs_coef_base() in the chain
+ * loop, else 0) */
+__device__ __forceinline__ float logscore_beta(const PriorArgs &p, float beta, uint32_t kb = 0u) {
+  return p.lg_ab + p.am1 * bbs_logf_at(beta, kb) + p.bm1 * bbs_logf_at(1.0f - beta, kb) -
          p.lg_a - p.lg_b;
 }
 

@@ -233,15 +233,20 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   uint32_t best_step = *a.best_step;
 
   /* the Philox blocks of 64 consecutive steps, one step per lane,
-   * computed on the vector unit; each step reads its 8 words by readlane */
+   * computed on the vector unit; each step reads its 8 words by readlane.
+   * The beta jumps (a normal draw, in double) of the same 64 steps are
+   * computed once per block, one per lane, whatever the move type: a beta
+   * move reads its own, other moves ignore theirs. */
   uint32_t rbase = a.step_begin;
   bbs_u32x4 ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
   bbs_u32x4 rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+  float jumps = bbs_beta_jump(rb, a.probs);
   for (uint32_t step = a.step_begin; step < a.step_end; step++) {
     if (step - rbase == 64u) {
       rbase = step;
       ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
       rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+      jumps = bbs_beta_jump_at(rb, a.probs, bbs_coef_base());
     }
     bbs_u32x4 wa, wb;
     const uint32_t sl = step - rbase;
@@ -250,7 +255,9 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       wa.x[i] = readlane32(ra.x[i], sl);
       wb.x[i] = readlane32(rb.x[i], sl);
     }
-    const bbs_raw_move raw = bbs_move_from_blocks(wa, wb, N, a.probs);
+    bbs_raw_move raw = bbs_move_without_jump(wa, wb, N, a.probs);
+    if (raw.move_type == BBS_MOVE_CHANGE_BETA)
+      raw.beta_jump = __uint_as_float(readlane32(__float_as_uint(jumps), sl));
     /* Realisation (src/MutationTree.hpp:323-363). Without a trace only what
      * the drawn move type uses is computed (a beta move needs no tree data,
      * prune and reattach a random non-descendant, a node swap the four rows,
@@ -332,9 +339,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     float prop_prior = cur_prior;
     if (m.type == BBS_MOVE_CHANGE_BETA) {
       prop_beta = nb;
-      Lp.L01 = bbs_logf(nb);
-      Lp.L11 = bbs_logf_of_double(1.0 - (double)nb);
-      prop_prior = logscore_beta(a.prior, nb);
+      const uint32_t kb = bbs_coef_base();
+      Lp.L01 = bbs_logf_at(nb, kb);
+      Lp.L11 = bbs_logf_of_double_at(1.0 - (double)nb, kb);
+      prop_prior = logscore_beta(a.prior, nb, kb);
     }
     uint64_t changed[NS];
     float prop_raw;

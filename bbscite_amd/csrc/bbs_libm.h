@@ -23,13 +23,6 @@
 
 #include "bbs_rng.h"
 
-/* Lookup tables: constant memory in the device pass, static data on the host. */
-#if defined(__HIP_DEVICE_COMPILE__)
-#define BBS_TABLE static __constant__ const
-#else
-#define BBS_TABLE static const
-#endif
-
 BBS_HD float bbs_f32_from_bits(uint32_t u) {
   union {
     uint32_t u;
@@ -113,9 +106,14 @@ BBS_HD float bbs_expf(float x) {
 
 /* ---------------------------------------------------------------- logf */
 
-BBS_HD float bbs_logf(float x) {
-  const double A0 = -0x1.00ea348b88334p-2, A1 = 0x1.5575b0be00b6ap-2,
-               A2 = -0x1.ffffef20a4123p-2, LN2 = 0x1.62e42fefa39efp-1;
+/* A0, A1, A2 of the cubic and ln 2 (the chain loop reads them at
+ * bbs_coef_base(): there bbs_logf runs on beta moves only). */
+BBS_TABLE double bbs_logf_k[4] = {-0x1.00ea348b88334p-2, 0x1.5575b0be00b6ap-2,
+                                  -0x1.ffffef20a4123p-2, 0x1.62e42fefa39efp-1};
+
+BBS_HD float bbs_logf_at(float x, uint32_t kb) {
+  const double A0 = bbs_logf_k[kb], A1 = bbs_logf_k[kb + 1u], A2 = bbs_logf_k[kb + 2u],
+               LN2 = bbs_logf_k[kb + 3u];
   const uint32_t OFF = 0x3f330000u;
   uint32_t ix = bbs_f32_bits(x);
   if (ix == 0x3f800000u) return 0.0f;
@@ -145,18 +143,22 @@ BBS_HD float bbs_logf(float x) {
   return (float)y;
 }
 
+BBS_HD float bbs_logf(float x) { return bbs_logf_at(x, 0u); }
+
 /* ------------------------------------------------- (float)log(double) */
 
 /* (float)log(x) for double x >= 0: log rounded to double, then to float,
  * the two roundings the reference's (float)std::log(double) performs. */
-BBS_HD float bbs_logf_of_double(double x) {
+BBS_HD float bbs_logf_of_double_at(double x, uint32_t kb) {
   if (x == 0.0) return bbs_f32_from_bits(0xff800000u);
   if (x == 1.0) return 0.0f;
   if (!(x > 0.0)) return bbs_f32_from_bits(0x7fc00000u);
   if (x > 0x1.fffffffffffffp+1023) return bbs_f32_from_bits(0x7f800000u);
   double lo;
-  double hi = bbs_log_dd(x, &lo);
+  double hi = bbs_log_dd_at(x, &lo, kb);
   return (float)hi;
 }
+
+BBS_HD float bbs_logf_of_double(double x) { return bbs_logf_of_double_at(x, 0u); }
 
 #endif /* BBS_LIBM_H */

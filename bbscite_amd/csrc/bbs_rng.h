@@ -37,6 +37,35 @@
 #define BBS_HD static inline
 #endif
 
+/* Lookup tables: constant memory in the device pass, static data on the host. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BBS_TABLE static __constant__ const
+#else
+#define BBS_TABLE static const
+#endif
+
+/*
+ * Index base of the coefficient tables below: zero, but opaque to the device
+ * compiler. Double-precision coefficients written as literals cannot be
+ * instruction operands, so the compiler materialises each in a VGPR pair and
+ * hoists it out of the chain loop, where the coefficients of the beta-move
+ * code (used on about a tenth of the steps) held some 60 VGPRs for the whole
+ * launch. Read from a table at this base, they are scalar loads issued where
+ * they are used. The functions taking an index base `kb` are called with
+ * bbs_coef_base() inside the chain loop and with a literal 0 elsewhere (the
+ * host, and kernels that use them once per item), where the compiler folds
+ * the table reads back into constants. Both read the same values.
+ */
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint32_t bbs_coef_base(void) {
+  uint32_t z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  return z;
+}
+#else
+BBS_HD uint32_t bbs_coef_base(void) { return 0u; }
+#endif
+
 /* Counter word 3: what a Philox block is used for. */
 enum {
   BBS_PURPOSE_MOVE_A = 0u,   /* move type, v, w, u_desc            */
@@ -116,15 +145,24 @@ BBS_HD uint64_t bbs_f64_bits(double d) {
   return c.u;
 }
 
+/* ln 2 as a double-double, sqrt(2), then the atanh series coefficients
+ * 1/25, 1/23, ..., 1/3 of bbs_log_dd, highest first. */
+BBS_TABLE double bbs_log_dd_k[15] = {
+    0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56, 0x1.6a09e667f3bcdp+0,
+    1.0 / 25.0, 1.0 / 23.0, 1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0,
+    1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0,
+    1.0 / 5.0, 1.0 / 3.0};
+
 /*
  * Natural log of x in (0, inf) as a double-double (hi + lo), from
  * log(x) = e*ln2 + 2*atanh(s), s = (m-1)/(m+1), m in [sqrt(1/2), sqrt(2)).
  * Relative error below 2^-58. Only IEEE +,-,*,/ and fma are used.
- * Subnormal inputs are scaled by 2^54 first.
+ * Subnormal inputs are scaled by 2^54 first. kb: the coefficients' index
+ * base, 0 or bbs_coef_base() (the _at functions below all take one).
  */
-BBS_HD double bbs_log_dd(double x, double *lo_out) {
-  const double LN2_HI = 0x1.62e42fefa39efp-1;
-  const double LN2_LO = 0x1.abc9e3b39803fp-56;
+BBS_HD double bbs_log_dd_at(double x, double *lo_out, uint32_t kb) {
+  const double LN2_HI = bbs_log_dd_k[kb];
+  const double LN2_LO = bbs_log_dd_k[kb + 1u];
   uint64_t ix = bbs_f64_bits(x);
   int e = 0;
   if ((ix >> 52) == 0) { /* subnormal */
@@ -135,7 +173,7 @@ BBS_HD double bbs_log_dd(double x, double *lo_out) {
   e += (int)((ix >> 52) & 0x7ff) - 1023;
   double m = bbs_f64_from_bits((ix & 0x000fffffffffffffull) |
                                0x3ff0000000000000ull); /* [1,2) */
-  if (m > 0x1.6a09e667f3bcdp+0) {                      /* > sqrt(2) */
+  if (m > bbs_log_dd_k[kb + 2u]) {                     /* > sqrt(2) */
     m = m * 0.5;
     e += 1;
   }
@@ -149,18 +187,8 @@ BBS_HD double bbs_log_dd(double x, double *lo_out) {
   double s_lo = rem / d_hi;
   /* tail: 2 s^3 (1/3 + s^2/5 + s^4/7 + ...) */
   double z = s_hi * s_hi;
-  double p = 1.0 / 25.0;
-  p = fma(p, z, 1.0 / 23.0);
-  p = fma(p, z, 1.0 / 21.0);
-  p = fma(p, z, 1.0 / 19.0);
-  p = fma(p, z, 1.0 / 17.0);
-  p = fma(p, z, 1.0 / 15.0);
-  p = fma(p, z, 1.0 / 13.0);
-  p = fma(p, z, 1.0 / 11.0);
-  p = fma(p, z, 1.0 / 9.0);
-  p = fma(p, z, 1.0 / 7.0);
-  p = fma(p, z, 1.0 / 5.0);
-  p = fma(p, z, 1.0 / 3.0);
+  double p = bbs_log_dd_k[kb + 3u];
+  for (uint32_t i = 4u; i < 15u; i++) p = fma(p, z, bbs_log_dd_k[kb + i]);
   double tail = 2.0 * (s_hi * z) * p;
   /* log(m) = 2 s_hi + (2 s_lo + tail) */
   double lm_hi = 2.0 * s_hi;
@@ -180,6 +208,8 @@ BBS_HD double bbs_log_dd(double x, double *lo_out) {
   return hi;
 }
 
+BBS_HD double bbs_log_dd(double x, double *lo_out) { return bbs_log_dd_at(x, lo_out, 0u); }
+
 /* sqrt(x) for x >= 0 from Newton steps on 1/sqrt(x); within 1 ulp. */
 BBS_HD double bbs_sqrt_det(double x) {
   if (!(x > 0.0)) return 0.0;
@@ -198,40 +228,31 @@ BBS_HD double bbs_sqrt_det(double x) {
   return s * bbs_f64_from_bits((uint64_t)(1023 + half) << 52);
 }
 
+/* 2 pi, then the Taylor coefficients 1/24!, 1/22!, ..., 1/2!, 1 of cos and
+ * 1/25!, 1/23!, ..., 1/3!, 1 of sin, highest first. */
+BBS_TABLE double bbs_cos_sin_k[27] = {
+    0x1.921fb54442d18p+2,
+    1.0 / 620448401733239439360000.0, 1.0 / 1124000727777607680000.0,
+    1.0 / 2432902008176640000.0, 1.0 / 6402373705728000.0,
+    1.0 / 20922789888000.0, 1.0 / 87178291200.0, 1.0 / 479001600.0,
+    1.0 / 3628800.0, 1.0 / 40320.0, 1.0 / 720.0, 1.0 / 24.0, 1.0 / 2.0, 1.0,
+    1.0 / 15511210043330985984000000.0, 1.0 / 25852016738884976640000.0,
+    1.0 / 51090942171709440000.0, 1.0 / 121645100408832000.0,
+    1.0 / 355687428096000.0, 1.0 / 1307674368000.0, 1.0 / 6227020800.0,
+    1.0 / 39916800.0, 1.0 / 362880.0, 1.0 / 5040.0, 1.0 / 120.0, 1.0 / 6.0, 1.0};
+
 /* cos(2*pi*k/2^32) for a 32-bit k. */
-BBS_HD double bbs_cos_2pi_u32(uint32_t k) {
-  const double TWO_PI = 0x1.921fb54442d18p+2;
+BBS_HD double bbs_cos_2pi_u32_at(uint32_t k, uint32_t kb) {
+  const double TWO_PI = bbs_cos_sin_k[kb];
   uint32_t q = k >> 30;
   double f = (double)(k & 0x3fffffffu) * 0x1p-32; /* [0, 1/4) exact */
   double t = TWO_PI * f;                          /* [0, pi/2) */
   double t2 = t * t;
   /* Taylor series of cos and sin to t^24 / t^25 (error < 1e-19) */
-  double c = 1.0 / 620448401733239439360000.0;
-  c = fma(c, -t2, 1.0 / 1124000727777607680000.0);
-  c = fma(c, -t2, 1.0 / 2432902008176640000.0);
-  c = fma(c, -t2, 1.0 / 6402373705728000.0);
-  c = fma(c, -t2, 1.0 / 20922789888000.0);
-  c = fma(c, -t2, 1.0 / 87178291200.0);
-  c = fma(c, -t2, 1.0 / 479001600.0);
-  c = fma(c, -t2, 1.0 / 3628800.0);
-  c = fma(c, -t2, 1.0 / 40320.0);
-  c = fma(c, -t2, 1.0 / 720.0);
-  c = fma(c, -t2, 1.0 / 24.0);
-  c = fma(c, -t2, 1.0 / 2.0);
-  c = fma(c, -t2, 1.0);
-  double s = 1.0 / 15511210043330985984000000.0;
-  s = fma(s, -t2, 1.0 / 25852016738884976640000.0);
-  s = fma(s, -t2, 1.0 / 51090942171709440000.0);
-  s = fma(s, -t2, 1.0 / 121645100408832000.0);
-  s = fma(s, -t2, 1.0 / 355687428096000.0);
-  s = fma(s, -t2, 1.0 / 1307674368000.0);
-  s = fma(s, -t2, 1.0 / 6227020800.0);
-  s = fma(s, -t2, 1.0 / 39916800.0);
-  s = fma(s, -t2, 1.0 / 362880.0);
-  s = fma(s, -t2, 1.0 / 5040.0);
-  s = fma(s, -t2, 1.0 / 120.0);
-  s = fma(s, -t2, 1.0 / 6.0);
-  s = fma(s, -t2, 1.0);
+  double c = bbs_cos_sin_k[kb + 1u];
+  for (uint32_t i = 2u; i < 14u; i++) c = fma(c, -t2, bbs_cos_sin_k[kb + i]);
+  double s = bbs_cos_sin_k[kb + 14u];
+  for (uint32_t i = 15u; i < 27u; i++) s = fma(s, -t2, bbs_cos_sin_k[kb + i]);
   s = s * t;
   switch (q) {
   case 0:
@@ -245,13 +266,19 @@ BBS_HD double bbs_cos_2pi_u32(uint32_t k) {
   }
 }
 
+BBS_HD double bbs_cos_2pi_u32(uint32_t k) { return bbs_cos_2pi_u32_at(k, 0u); }
+
 /* Standard normal by Box-Muller from two 32-bit words. */
-BBS_HD double bbs_normal_from_u32(uint32_t a, uint32_t b) {
+BBS_HD double bbs_normal_from_u32_at(uint32_t a, uint32_t b, uint32_t kb) {
   double u1 = ((double)a + 1.0) * 0x1p-32; /* (0, 1] */
   double lo;
-  double l = bbs_log_dd(u1, &lo);
+  double l = bbs_log_dd_at(u1, &lo, kb);
   double r = bbs_sqrt_det(-2.0 * l);
-  return r * bbs_cos_2pi_u32(b);
+  return r * bbs_cos_2pi_u32_at(b, kb);
+}
+
+BBS_HD double bbs_normal_from_u32(uint32_t a, uint32_t b) {
+  return bbs_normal_from_u32_at(a, b, 0u);
 }
 
 /*
@@ -290,8 +317,16 @@ BBS_HD uint32_t bbs_move_type_of(float u, bbs_move_probs p) {
  * u_accept). The normal is evaluated only for beta-change moves: the jump
  * does not influence any other move (src/MutationTree.hpp:128-130).
  */
-BBS_HD bbs_raw_move bbs_move_from_blocks(bbs_u32x4 a, bbs_u32x4 b, uint32_t n_nodes,
-                                         bbs_move_probs p) {
+/* The beta jump of a beta-change move whose block B is b. */
+BBS_HD float bbs_beta_jump_at(bbs_u32x4 b, bbs_move_probs p, uint32_t kb) {
+  return (float)(bbs_normal_from_u32_at(b.x[1], b.x[2], kb) * (double)p.beta_jump_sd);
+}
+
+BBS_HD float bbs_beta_jump(bbs_u32x4 b, bbs_move_probs p) { return bbs_beta_jump_at(b, p, 0u); }
+
+/* Everything of bbs_move_from_blocks but the beta jump (left 0). */
+BBS_HD bbs_raw_move bbs_move_without_jump(bbs_u32x4 a, bbs_u32x4 b, uint32_t n_nodes,
+                                          bbs_move_probs p) {
   bbs_raw_move m;
   m.move_type = bbs_move_type_of(bbs_u01f(a.x[0]), p);
   m.raw_v = bbs_uint_below(a.x[1], n_nodes - 1u);
@@ -301,11 +336,14 @@ BBS_HD bbs_raw_move bbs_move_from_blocks(bbs_u32x4 a, bbs_u32x4 b, uint32_t n_no
   m.raw_descendant_of_v = bbs_u01f(a.x[3]);
   m.raw_nondescendant_of_v = bbs_u01f(b.x[0]);
   m.beta_jump = 0.0f;
-  if (m.move_type == 0u) {
-    m.beta_jump =
-        (float)(bbs_normal_from_u32(b.x[1], b.x[2]) * (double)p.beta_jump_sd);
-  }
   m.acceptance_level = bbs_u01f(b.x[3]);
+  return m;
+}
+
+BBS_HD bbs_raw_move bbs_move_from_blocks(bbs_u32x4 a, bbs_u32x4 b, uint32_t n_nodes,
+                                         bbs_move_probs p) {
+  bbs_raw_move m = bbs_move_without_jump(a, b, n_nodes, p);
+  if (m.move_type == 0u) m.beta_jump = bbs_beta_jump(b, p);
   return m;
 }
 
