@@ -142,6 +142,31 @@ __device__ __forceinline__ uint32_t parent_of(const uint64_t (&dm)[NS][NS],
   return result;
 }
 
+/* glibc's expf (bbs_expf) with its 32-entry table across the lanes of a
+ * register pair, lane i < 32 holding entry i. The chain's acceptance test
+ * takes expf once per step of a uniform argument, so the entry comes by two
+ * readlanes instead of two dependent memory loads (the table's address,
+ * then the entry). */
+struct Exp2Lanes {
+  uint32_t lo, hi;
+};
+
+__device__ __forceinline__ Exp2Lanes exp2_lanes() {
+  const uint64_t e = bbs_exp2f_table[lane_id() & 31u];
+  return Exp2Lanes{(uint32_t)e, (uint32_t)(e >> 32)};
+}
+
+/* bbs_expf(x) for x equal on every lane */
+__device__ __forceinline__ float expf_lanes(float x, const Exp2Lanes &t) {
+  float special;
+  double r;
+  uint64_t ki;
+  if (bbs_expf_head(x, &special, &r, &ki)) return special;
+  const uint32_t i = rfl((uint32_t)ki & 31u);
+  const uint64_t e = ((uint64_t)readlane32(t.hi, i) << 32) | readlane32(t.lo, i);
+  return bbs_expf_tail(r, ki, e);
+}
+
 /* popcount(x) + acc as one v_bcnt_u32_b32 (the compiler would otherwise
  * re-associate a chain of popcounts into separate counts plus adds). */
 __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) {

@@ -241,6 +241,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   bbs_u32x4 ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
   bbs_u32x4 rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
   float jumps = bbs_beta_jump(rb, a.probs);
+  const Exp2Lanes exp2t = exp2_lanes();
   for (uint32_t step = a.step_begin; step < a.step_end; step++) {
     if (step - rbase == 64u) {
       rbase = step;
@@ -376,7 +377,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     float corr = 1.0f;
     if (m.type == BBS_MOVE_SWAP_SUBTREES && m.related)
       corr = (float)nd_v / (float)nd_w;
-    const float acc_prob = corr * bbs_expf((prop_score - cur_score) * a.gamma);
+    const float acc_prob = corr * expf_lanes((prop_score - cur_score) * a.gamma, exp2t);
     const bool accept = acc_prob > raw.acceptance_level;
     const bool new_max = (a.is_first_chain && step == 0u) || prop_score > best_score;
 
@@ -1076,13 +1077,21 @@ hipError_t launch_valu_peak(const uint32_t *in, uint32_t *out, uint32_t blocks,
 
 /* ------------------------------------------------------------ libm probe */
 
-/* Device values of the three libm restatements, for comparison with glibc. */
+/* Device values of the three libm restatements, for comparison with glibc.
+ * expf is taken as the chain takes it: expf_lanes of a wave-uniform
+ * argument, each lane's value in turn. */
 __global__ __launch_bounds__(256) void libm_kernel(const float *x, uint32_t n,
                                                    float *e, float *l, float *l1m) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const Exp2Lanes t = exp2_lanes();
+  const float v = i < n ? x[i] : 0.0f;
+  float ev = 0.0f;
+  for (uint32_t j = 0; j < 64u; j++) {
+    const float r = expf_lanes(__uint_as_float(readlane32(__float_as_uint(v), j)), t);
+    if (lane_id() == j) ev = r;
+  }
   if (i >= n) return;
-  const float v = x[i];
-  e[i] = bbs_expf(v);
+  e[i] = ev;
   l[i] = bbs_logf(v);
   l1m[i] = bbs_logf_of_double(1.0 - (double)v);
 }

@@ -74,26 +74,49 @@ BBS_TABLE double bbs_logf_logc[16] = {
     0x1.526e57720db08p-3,  0x1.bc2860d22477p-3,   0x1.1058bc8a07ee1p-2,
     0x1.4043057b6ee09p-2};
 
-BBS_HD float bbs_expf(float x) {
-  const double C0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32;
-  const double C1 = 0x1.ebfce50fac4f3p-3 / 32 / 32;
-  const double C2 = 0x1.62e42ff0c52d6p-1 / 32;
+/* expf in two halves around its one table read, so that a caller may keep
+ * the table elsewhere (the chain kernel holds it across the lanes of a
+ * register pair). bbs_expf_head returns 1 with *special = expf(x) for
+ * |x| >= 88, inf and nan; otherwise 0 with the reduction (r, ki), whose
+ * table entry is bbs_exp2f_table[ki % 32]. bbs_expf_tail finishes from
+ * that entry. */
+BBS_HD int bbs_expf_head(float x, float *special, double *r_out, uint64_t *ki_out) {
   const double INV_LN2_N = 0x1.71547652b82fep+0 * 32;
   const double SHIFT = 0x1.8p+52;
   uint32_t ix = bbs_f32_bits(x);
   uint32_t abstop = (ix >> 20) & 0x7ffu;
   if (abstop >= 0x42bu) { /* |x| >= 88 or x is inf/nan */
-    if (ix == 0xff800000u) return 0.0f;
-    if (abstop >= 0x7f8u) return x + x;
-    if (x > 0x1.62e42ep6f) return bbs_f32_from_bits(0x7f800000u);
-    if (x < -0x1.9fe368p6f) return 0.0f;
+    if (ix == 0xff800000u) {
+      *special = 0.0f;
+      return 1;
+    }
+    if (abstop >= 0x7f8u) {
+      *special = x + x;
+      return 1;
+    }
+    if (x > 0x1.62e42ep6f) {
+      *special = bbs_f32_from_bits(0x7f800000u);
+      return 1;
+    }
+    if (x < -0x1.9fe368p6f) {
+      *special = 0.0f;
+      return 1;
+    }
   }
   double xd = (double)x;
   double kd = fma(INV_LN2_N, xd, SHIFT);
   uint64_t ki = bbs_f64_bits(kd);
   kd -= SHIFT;
-  double r = fma(INV_LN2_N, xd, -kd);
-  uint64_t t = bbs_exp2f_table[ki % 32u];
+  *r_out = fma(INV_LN2_N, xd, -kd);
+  *ki_out = ki;
+  return 0;
+}
+
+BBS_HD float bbs_expf_tail(double r, uint64_t ki, uint64_t entry) {
+  const double C0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32;
+  const double C1 = 0x1.ebfce50fac4f3p-3 / 32 / 32;
+  const double C2 = 0x1.62e42ff0c52d6p-1 / 32;
+  uint64_t t = entry;
   t += ki << (52 - 5);
   double s = bbs_f64_from_bits(t);
   double z = fma(C0, r, C1);
@@ -102,6 +125,14 @@ BBS_HD float bbs_expf(float x) {
   y = fma(z, r2, y);
   y = y * s;
   return (float)y;
+}
+
+BBS_HD float bbs_expf(float x) {
+  float special;
+  double r;
+  uint64_t ki;
+  if (bbs_expf_head(x, &special, &r, &ki)) return special;
+  return bbs_expf_tail(r, ki, bbs_exp2f_table[ki % 32u]);
 }
 
 /* ---------------------------------------------------------------- logf */
