@@ -440,6 +440,41 @@ __device__ __forceinline__ float ordered_sum(uint32_t n_cells, WaveLds &S) {
   return rfl_f(T);
 }
 
+/* A chain split over several waves of one workgroup by cells: this wave
+ * holds cells [c0, c0 + its count) (one cell slot, its own pair-score table
+ * and rows in LDS). Each scoring's per-cell maxima of all waves meet in
+ * `shared` (n_total floats, +0.0 after them up to 128), and every wave then
+ * takes the same ordered sum, so all waves make the same decisions. Two
+ * shared buffers alternate between consecutive scorings: a wave writes a
+ * buffer again only after the next scoring's barrier, which the other waves
+ * reach only after their reads of it. */
+struct CellSplit {
+  float *shared;    /* the buffer of this scoring (2 x 128 floats in all) */
+  uint32_t c0, n_total;
+};
+
+/* ((0 + x[0]) + x[1]) + ... over n <= 128 floats of LDS, +0.0 past n, padded
+ * to the next multiple of 16 */
+__device__ __forceinline__ float ordered_sum_n(uint32_t n, const float *x) {
+  if (n <= 64u) {
+    return n <= 16u ? sum_padded<4>(x) : n <= 32u ? sum_padded<8>(x)
+         : n <= 48u ? sum_padded<12>(x) : sum_padded<16>(x);
+  }
+  return n <= 80u ? sum_padded<20>(x) : n <= 96u ? sum_padded<24>(x)
+       : n <= 112u ? sum_padded<28>(x) : sum_padded<32>(x);
+}
+
+/* finish_sum of a cell-split chain (one cell slot per wave): this wave's
+ * maxima into the shared buffer, a workgroup barrier, the ordered sum over
+ * every wave's cells. */
+__device__ __forceinline__ float finish_sum_split(float best, bool nan0, uint32_t n_cells,
+                                                  const CellSplit &sp) {
+  const uint32_t lane = lane_id();
+  if (lane < n_cells) sp.shared[sp.c0 + lane] = nan0 ? __int_as_float(0x7fc00000) : best;
+  __syncthreads();
+  return rfl_f(ordered_sum_n(sp.n_total, sp.shared));
+}
+
 /* Per-cell maxima -> ordered cell sum (src/TreeScorer.hpp:198-205).
  * best[j]/nan0[j] of the full slots, and for the paired slot the maxima of
  * the two lane halves. Cells past n_cells store +0.0: the running sum starts
@@ -704,16 +739,22 @@ __device__ __forceinline__ float mfma_maxima(const MfmaCells<CS, W> &mc, uint32_
  * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
  * or 2 only), n01/n11 of the first maximal node are returned per slot.
  */
-template <int NS, int CS, int W, bool COUNTS, bool TAB = false>
+template <int NS, int CS, int W, bool COUNTS, bool TAB = false, bool SPL = false>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
            uint32_t n_cells, const LogProbs &L, WaveLds &S,
-           uint32_t (&arg01)[2], uint32_t (&arg11)[2], TableView tab = TableView{nullptr, 0u, 0u}) {
+           uint32_t (&arg01)[2], uint32_t (&arg11)[2], TableView tab = TableView{nullptr, 0u, 0u},
+           const CellSplit &sp = CellSplit{nullptr, 0u, 0u}) {
   store_rows<NS>(dm, n_nodes, S);
   float best[2];
   bool nan0[2];
   node_maxima<NS, CS, W, COUNTS, TAB>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
-  return finish_sum<CS>(best, nan0, n_cells, S);
+  if constexpr (SPL) {
+    static_assert(CS == 1, "a split chain holds one cell slot per wave");
+    return finish_sum_split(best[0], nan0[0], n_cells, sp);
+  } else {
+    return finish_sum<CS>(best, nan0, n_cells, S);
+  }
 }
 
 /* Lowest set bit of a node mask held uniformly by the wave (bit v = node v),
@@ -956,17 +997,23 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
  * not depend on the order the nodes are visited in (the first-wins rule
  * only matters for a NaN at node 0, kept by the flags), so the result is
  * bit-identical to the full scorer. */
-template <int NS, int CS, int W>
+template <int NS, int CS, int W, bool SPL = false>
 __device__ __forceinline__ float score_changed(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
                                                uint32_t n_nodes, uint32_t n_cells,
-                                               const LogProbs &L, WaveLds &S, TableView tab) {
+                                               const LogProbs &L, WaveLds &S, TableView tab,
+                                               const CellSplit &sp = CellSplit{nullptr, 0u, 0u}) {
   const float ninf = __int_as_float(0xff800000);
   float R[2] = {ninf, ninf}, U[2] = {ninf, ninf};
   bool nan0[2] = {false, false};
   changed_nodes<NS, CS, W, false>(changed, c, L, S, tab, R, nan0);
   unchanged_nodes<NS, CS>((changed[0] & 1ull) != 0ull, n_nodes, tab, S, U, nan0);
   float best[2] = {vmax(R[0], U[0]), vmax(R[1], U[1])};
-  return finish_sum<CS>(best, nan0, n_cells, S);
+  if constexpr (SPL) {
+    static_assert(CS == 1, "a split chain holds one cell slot per wave");
+    return finish_sum_split(best[0], nan0[0], n_cells, sp);
+  } else {
+    return finish_sum<CS>(best, nan0, n_cells, S);
+  }
 }
 
 /* Constants of the beta prior (src/TreeScorer.hpp:108-111,122-126). */

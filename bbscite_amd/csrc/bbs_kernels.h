@@ -92,6 +92,12 @@ void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
  * incremental chain wave (chain_table_bytes). */
 hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
                         hipStream_t stream);
+/* A chain split over `split` (2 or 4) waves by cells, for launches with
+ * fewer chains than SIMDs: LDS bytes of its workgroup, 0 if the split does
+ * not fit (or would leave a wave without cells). */
+uint32_t chain_split_bytes(uint32_t n_nodes, uint32_t n_cells, uint32_t split);
+hipError_t launch_chain_split(const ChainArgs &a, bool trace, uint32_t split,
+                              hipStream_t stream);
 /* LDS one workgroup may allocate on gfx950: the whole 160 KB of a CU. */
 constexpr uint32_t kMaxGroupLds = 163840u;
 

@@ -176,10 +176,24 @@ struct ChainView {
  * count pairs on the matrix cores where the dataset fills its tiles (the
  * plain kernel; the batched kernel, mostly small datasets, keeps its
  * registers for the vector path). */
-template <int NS, int CS, int W, bool TRACE, bool INCR, bool MFMA>
-__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, TableView tab) {
+template <int NS, int CS, int W, bool TRACE, bool INCR, bool MFMA, bool SPL = false>
+__device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, TableView tab,
+                                          const CellSplit &split = CellSplit{nullptr, 0u, 0u},
+                                          bool leader = true) {
+  static_assert(!SPL || (CS == 1 && INCR && !MFMA), "a split chain: one cell slot, table");
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
+  /* SPL: this wave scores the cells a.cells[0, a.n_cells) of a chain split
+   * over the waves of its workgroup; every wave runs the same move, and only
+   * the leader writes to memory. Each scoring sums in the next of the two
+   * shared buffers (split.shared is the first). */
+  uint32_t flip = 0u;
+  auto next_split = [&]() {
+    CellSplit sp = split;
+    sp.shared = split.shared + 128u * (flip & 1u);
+    flip++;
+    return sp;
+  };
 
   Cells<CS, W> cells;
   load_cells<CS, W>(a.cells, a.n_cells, cells);
@@ -217,8 +231,9 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       store_rows<NS>(dm, N, lds);
       t = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, L, lds, tab);
     } else {
-      t = score_tree<NS, CS, W, false, INCR>(dm, cells, N, a.n_cells, L, lds, unused01,
-                                             unused11, INCR ? tab : TableView{nullptr, 0u, 0u});
+      t = score_tree<NS, CS, W, false, INCR, SPL>(dm, cells, N, a.n_cells, L, lds, unused01,
+                                                  unused11, INCR ? tab : TableView{nullptr, 0u, 0u},
+                                                  SPL ? next_split() : split);
     }
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
@@ -358,7 +373,8 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       }
       store_pen<NS>(diff, N, lds);
       store_rows<NS>(pdm, N, lds);
-      prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
+      prop_raw = score_changed<NS, CS, W, SPL>(changed, cells, N, a.n_cells, Lp, lds, tab,
+                                               SPL ? next_split() : split);
     } else {
       /* beta moves (or full scoring): with INCR the proposal's pair scores
        * go straight into the table, since most beta moves are accepted; a
@@ -367,9 +383,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
         store_rows<NS>(pdm, N, lds);
         prop_raw = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, Lp, lds, tab);
       } else {
-        prop_raw = score_tree<NS, CS, W, false, INCR>(pdm, cells, N, a.n_cells, Lp, lds,
-                                                      unused01, unused11,
-                                                      INCR ? tab : TableView{nullptr, 0u, 0u});
+        prop_raw = score_tree<NS, CS, W, false, INCR, SPL>(pdm, cells, N, a.n_cells, Lp, lds,
+                                                           unused01, unused11,
+                                                           INCR ? tab : TableView{nullptr, 0u, 0u},
+                                                           SPL ? next_split() : split);
       }
     }
     const float prop_score = prop_raw + prop_prior;
@@ -381,7 +398,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     const bool accept = acc_prob > raw.acceptance_level;
     const bool new_max = (a.is_first_chain && step == 0u) || prop_score > best_score;
 
-    if (TRACE && a.trace != nullptr && lane == 0u) {
+    if (TRACE && a.trace != nullptr && lane == 0u && leader) {
       bbs_step_trace tr;
       tr.move_type = m.type;
       tr.v = v;
@@ -403,7 +420,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
 #pragma unroll
       for (int k = 0; k < NS; k++) {
         const uint32_t x = lane + 64u * (uint32_t)k;
-        if (x < N) {
+        if (x < N && leader) {
 #pragma unroll
           for (int wd = 0; wd < NS; wd++) {
             a.best_am[2u * x + (uint32_t)wd] = pam[k][wd];
@@ -450,6 +467,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     }
   }
 
+  if (!leader) return;
 #pragma unroll
   for (int k = 0; k < NS; k++) {
     const uint32_t x = lane + 64u * (uint32_t)k;
@@ -546,6 +564,60 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
     WaveLds lds = store.view();
     run_chain<NS, CS, W, TRACE, false, false>(v, lds, TableView{nullptr, 0u, 0u});
   }
+}
+
+/* One chain per workgroup of SPLIT waves, split by cells: wave i scores
+ * cells [i * cs, (i + 1) * cs), cs = ceil(n_cells / SPLIT), with its own
+ * table; every wave computes the same move, acceptance and ordered cell sum.
+ * For launches with fewer chains than SIMDs, where a step's latency, not
+ * the device's width, sets the rate. Dynamic LDS: the two shared sum buffers
+ * (2 x 128 floats), then one ChainLds slab of cs cells per wave. */
+template <int NS, int W, bool TRACE, int SPLIT>
+__global__ __launch_bounds__(64 * SPLIT) void chain_split_kernel(ChainArgs a) {
+  extern __shared__ __align__(16) float dyn[];
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t cs = (a.n_cells + (uint32_t)SPLIT - 1u) / (uint32_t)SPLIT;
+  const uint32_t c0 = wave * cs;
+  const uint32_t mine = a.n_cells - c0 < cs ? a.n_cells - c0 : cs; /* >= 1 (host check) */
+  /* the sums past n_cells stay +0.0 */
+  for (uint32_t i = threadIdx.x; i < 256u; i += 64u * (uint32_t)SPLIT) dyn[i] = 0.0f;
+  __syncthreads();
+  const uint64_t rows = 2ull * chain * a.n_nodes;
+  ChainView v;
+  v.cells = a.cells + 4ull * c0;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + chain;
+  v.score = a.score + chain;
+  v.best_beta = a.best_beta + chain;
+  v.best_score = a.best_score + chain;
+  v.best_step = a.best_step + chain;
+  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
+                                          : nullptr;
+  v.seed = a.seed;
+  v.job = a.job;
+  v.chain = a.chain_offset + chain;
+  v.step_begin = a.step_begin;
+  v.step_end = a.step_end;
+  v.n_cells = mine;
+  v.n_nodes = a.n_nodes;
+  v.is_first_chain = a.chain_offset + chain == 0u;
+  v.score_init = a.score_init != 0u;
+  v.L00 = a.L00;
+  v.L10 = a.L10;
+  v.gamma = a.gamma;
+  v.prior = a.prior;
+  v.probs = a.probs;
+  const ChainLds l = ChainLds::of(a.n_nodes, cs, 1);
+  float *slab = dyn + 256u + wave * l.total;
+  WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
+  run_chain<NS, 1, W, TRACE, true, false, true>(
+      v, lds, TableView{slab + l.table, TableView::stride_for(a.n_nodes), mine},
+      CellSplit{dyn, c0, a.n_cells}, wave == 0u);
 }
 
 /* ---------------------------------------------------------- batched kernel */
@@ -1155,6 +1227,46 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
     return launch_chain_shape<NS_, CS_, W_>(a, trace, table_bytes, stream);
   BBS_SHAPES(X)
 #undef X
+  return hipErrorInvalidValue;
+}
+
+uint32_t chain_split_bytes(uint32_t n_nodes, uint32_t n_cells, uint32_t split) {
+  if (split != 2u && split != 4u) return 0u;
+  const uint32_t cs = (n_cells + split - 1u) / split;
+  if (cs == 0u || (split - 1u) * cs >= n_cells) return 0u; /* a wave without cells */
+  const uint32_t bytes = 4u * (256u + split * ChainLds::of(n_nodes, cs, 1).total);
+  return bytes <= kMaxGroupLds ? bytes : 0u;
+}
+
+template <int NS, int W>
+static hipError_t launch_chain_split_shape(const ChainArgs &a, bool trace, uint32_t split,
+                                           uint32_t bytes, hipStream_t stream) {
+  const dim3 grid(a.n_chains), block(64u * split);
+#define XS(SP_)                                                                \
+  if (split == SP_) {                                                          \
+    if (trace)                                                                 \
+      hipLaunchKernelGGL((chain_split_kernel<NS, W, true, SP_>), grid, block,  \
+                         bytes, stream, a);                                    \
+    else                                                                       \
+      hipLaunchKernelGGL((chain_split_kernel<NS, W, false, SP_>), grid, block, \
+                         bytes, stream, a);                                    \
+    return hipGetLastError();                                                  \
+  }
+  XS(2u) XS(4u)
+#undef XS
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_chain_split(const ChainArgs &a, bool trace, uint32_t split,
+                              hipStream_t stream) {
+  const uint32_t bytes = chain_split_bytes(a.n_nodes, a.n_cells, split);
+  if (bytes == 0u) return hipErrorInvalidValue;
+  int ns, cs, w;
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
+#define X2(NS_, W_)                                                            \
+  if (ns == NS_ && w == W_) return launch_chain_split_shape<NS_, W_>(a, trace, split, bytes, stream);
+  X2(1, 1) X2(1, 2) X2(2, 2) X2(2, 3) X2(2, 4)
+#undef X2
   return hipErrorInvalidValue;
 }
 

@@ -473,6 +473,32 @@ def plain_64x63_bench(with_cpu):
     return out
 
 
+def few_chains_bench():
+    """8 chains of a wide dataset (the plain launch with fewer chains than
+    SIMDs, as the command line runs it): one wave per chain against each chain
+    split over two waves by cells (the automatic choice above 64 cells)."""
+    out = {}
+    for G, C, steps in ((94, 95, 50_000), (127, 128, 30_000)):
+        m, _, _ = B.generate_dataset(2, G, C, 1e-4, 0.3, 0.15)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=8, n_steps=steps, seed=2, alpha_mean=1e-4, beta_mean=0.3,
+                             beta_sd=0.05, prob_beta_change=0.1)
+        row = {"chains": 8, "steps": steps}
+        best = {}
+        for k in (1, 2):
+            with B.Context(p, mut, kno, G) as ctx:
+                ctx.set_chain_split(k)
+                ms = ctx.run()
+                best[k] = ctx.best()[3]
+            row[f"{k}_wave_per_chain_steps_per_s" if k == 1 else f"{k}_waves_per_chain_steps_per_s"] = \
+                8 * steps / (ms / 1e3)
+        row["best_ll_identical"] = bool(np.float32(best[1]).tobytes() == np.float32(best[2]).tobytes())
+        out[f"{G}x{C}"] = row
+        log(f"[bench] {G}x{C} 8 chains: {row['1_wave_per_chain_steps_per_s']:.3e} steps/s one wave "
+            f"per chain, {row['2_waves_per_chain_steps_per_s']:.3e} split over two")
+    return out
+
+
 def layout_code(genes, cells):
     """Register layout (node slots, cell slots, gene words) as the library's
     shape_of picks it, as the code NS*100 + CS*10 + W."""
@@ -691,6 +717,7 @@ def main():
     check = None if a.no_cpu or snap is None else oracle_check(p, mut, kno, S, snap)
 
     small = None if a.no_small else plain_64x63_bench(not a.no_cpu)
+    few = None if a.no_small else few_chains_bench()
     score = None if a.no_score else scoring_bench(peak=peak)
     if score:
         log(f"[bench] scoring: {score['trees_per_s']:.3e} trees/s")
@@ -748,6 +775,7 @@ def main():
         "best_beta": best_beta,
         "oracle_check": check,
         "plain_64x63": small,
+        "few_chains": few,
         "scoring": score,
         "batched": batched,
         "serving_p99_ms": serving["p99_ms"] if serving else None,

@@ -119,6 +119,23 @@ int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
 #define BBS_SCORING_INCREMENTAL 2
 int bbs_set_scoring_mode(bbs_ctx *ctx, int mode);
 
+/* Waves per chain of the plain launch (no reference counterpart: a launch
+ * shape; the reference runs one pipelined chain step at a time,
+ * src/Application.hpp:398-496). 1: one wavefront per chain. 2 or 4: each
+ * chain runs on a workgroup of that many wavefronts that split its cells
+ * (ceil(n_cells / k) each) and compute the same move, acceptance and ordered
+ * cell sum, with one workgroup barrier per step; with over 64 cells (two
+ * cell slots in one wave) it shortens a step where the launch has fewer
+ * chains than the device has SIMDs. 0 (the default): 2 when the dataset has
+ * over 64 cells, 2 n_chains waves fit the device's SIMDs and the incremental
+ * scorer is in use, else 1. Results are
+ * bit-identical for every value; with BBS_SCORING_FULL a chain always has
+ * one wavefront (a split chain keeps the incremental scorer's table).
+ * BBS_ERR_ARG for other values, BBS_ERR_SIZE
+ * if the split does not fit this dataset (over the LDS of a workgroup or a
+ * wave without cells). */
+int bbs_set_chain_split(bbs_ctx *ctx, uint32_t waves_per_chain);
+
 /* Replaces Application::run_simulation (src/Application.hpp:197-219): runs
  * all chains for n_steps steps and blocks; *makespan_ms (nullable) receives
  * the device time in milliseconds, measured with HIP events. */

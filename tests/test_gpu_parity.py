@@ -165,15 +165,21 @@ CHAIN_CASES = [
 
 
 def _set_mode(ctx, mode):
+    """full: every pair rescored; incremental: the pair-score table, one
+    wave per chain; split2 / split4: the table with each chain split over 2
+    or 4 waves by cells."""
     try:
-        ctx.set_scoring_mode(mode)
+        ctx.set_scoring_mode("full" if mode == "full" else "incremental")
+        if mode != "full":
+            ctx.set_chain_split({"incremental": 1, "split2": 2, "split4": 4}[mode])
     except B.BbsError as e:
-        if mode == "incremental" and e.code == -2:
-            pytest.skip("no incremental scorer for this size (table over the LDS of a workgroup)")
+        if mode != "full" and e.code == -2:
+            pytest.skip(f"no {mode} scorer for this size (LDS of a workgroup, or a wave "
+                        "without cells)")
         raise
 
 
-@pytest.mark.parametrize("mode", ["full", "incremental"])
+@pytest.mark.parametrize("mode", ["full", "incremental", "split2", "split4"])
 @pytest.mark.parametrize("case", CHAIN_CASES, ids=lambda c: f"G{c[0]}C{c[1]}")
 def test_chain_trace_replay(case, mode):
     G, C, seed, R, S, pb, bm, bsd = case
@@ -232,21 +238,26 @@ def test_incremental_equals_full_over_long_runs(G, C):
     say(f"[incremental] G={G} C={C}")
     p, mut, kno = _chain_case(G, C, 12, 16, 4000, 0.1, beta_mean=0.3, beta_sd=0.05)
     out = {}
-    for mode in ("full", "incremental"):
+    for mode in ("full", "incremental", "split2", "split4"):
         with B.Context(p, mut, kno, G) as ctx:
-            _set_mode(ctx, mode)
+            try:
+                _set_mode(ctx, mode)
+            except pytest.skip.Exception:
+                continue
             ctx.begin()
             for n in (1000, 3000):
                 ctx.advance(n)
             out[mode] = (ctx.chain_results(), ctx.chain_states(), ctx.best())
-    (r0, s0, b0), (r1, s1, b1) = out["full"], out["incremental"]
-    for x, y in zip(r0, r1):
-        assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
-                              _bits(y) if y.dtype == np.float32 else y)
-    for x, y in zip(s0, s1):
-        assert np.array_equal(x, y)
-    assert np.array_equal(b0[0], b1[0]) and np.array_equal(b0[1], b1[1])
-    assert _bits(b0[3]) == _bits(b1[3])
+    r0, s0, b0 = out["full"]
+    assert len(out) >= 2
+    for mode, (r1, s1, b1) in out.items():
+        for x, y in zip(r0, r1):
+            assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                                  _bits(y) if y.dtype == np.float32 else y), mode
+        for x, y in zip(s0, s1):
+            assert np.array_equal(x, y), mode
+        assert np.array_equal(b0[0], b1[0]) and np.array_equal(b0[1], b1[1]), mode
+        assert _bits(b0[3]) == _bits(b1[3]), mode
 
 
 def test_advance_windows_equal_one_run():
@@ -354,6 +365,18 @@ def test_errors_are_reported():
         ctx.begin()
         with pytest.raises(B.BbsError):
             ctx.advance(11)
+        # waves per chain: 0, 1, 2 or 4; a split needs a cell for every wave
+        with pytest.raises(B.BbsError) as e:
+            ctx.set_chain_split(3)
+        assert e.value.code == B.ERR_ARG
+        ctx.set_chain_split(4)  # 4 cells, one per wave
+    m3 = _iid_matrix(2, 4, 3)
+    mut3, kno3 = B.encode_matrix(m3)
+    with B.Context(B.default_params(n_chains=2, n_steps=10), mut3, kno3, 4) as ctx:
+        with pytest.raises(B.BbsError) as e:
+            ctx.set_chain_split(4)  # 3 cells cannot feed 4 waves
+        assert e.value.code == B.ERR_SIZE
+        ctx.set_chain_split(2)
 
 
 def _mix(n_jobs, seed0, steps, chains_range=(2, 6)):
