@@ -166,6 +166,7 @@ _SIGS = {
     "bbs_set_trace": (C.c_int, [_P, _u32]),
     "bbs_set_scoring_mode": (C.c_int, [_P, C.c_int]),
     "bbs_set_chain_split": (C.c_int, [_P, C.c_uint32]),
+    "bbs_set_speculation": (C.c_int, [_P, C.c_uint32]),
     "bbs_run": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "bbs_begin": (C.c_int, [_P]),
     "bbs_advance": (C.c_int, [_P, _u32, C.POINTER(C.c_float)]),
@@ -448,6 +449,11 @@ class Context:
     def set_chain_split(self, waves_per_chain):
         """Waves per chain: 0 (auto), 1, 2 or 4 (results are identical)."""
         self._c(lib().bbs_set_chain_split(self.h, waves_per_chain))
+
+    def set_speculation(self, waves_per_chain):
+        """Speculative waves per chain: 0 (auto), 1 (off), 2, 4 or 8
+        (results are identical)."""
+        self._c(lib().bbs_set_speculation(self.h, waves_per_chain))
 
     def run(self):
         ms = C.c_float()

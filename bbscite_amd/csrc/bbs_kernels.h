@@ -98,6 +98,10 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
 uint32_t chain_split_bytes(uint32_t n_nodes, uint32_t n_cells, uint32_t split);
 hipError_t launch_chain_split(const ChainArgs &a, bool trace, uint32_t split,
                               hipStream_t stream);
+/* A chain run speculatively on k (2, 4 or 8) waves that propose consecutive
+ * steps at once: LDS bytes of its workgroup, 0 if it does not fit. */
+uint32_t chain_spec_bytes(uint32_t n_nodes, uint32_t n_cells, uint32_t k);
+hipError_t launch_chain_spec(const ChainArgs &a, bool trace, uint32_t k, hipStream_t stream);
 /* LDS one workgroup may allocate on gfx950: the whole 160 KB of a CU. */
 constexpr uint32_t kMaxGroupLds = 163840u;
 

@@ -146,6 +146,170 @@ __device__ __forceinline__ void apply_move(const MoveCtx<NS> &m,
   }
 }
 
+/* ------------------------------------------------------------ proposals */
+
+/* The raw move of the step held by lane sl of the Philox blocks (ra, rb) and
+ * the beta jumps of the same steps (src/RawMoveDistribution.hpp:97-155). */
+__device__ __forceinline__ bbs_raw_move draw_move(const bbs_u32x4 &ra, const bbs_u32x4 &rb,
+                                                  float jumps, uint32_t sl, uint32_t n_nodes,
+                                                  const bbs_move_probs &probs) {
+  bbs_u32x4 wa, wb;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    wa.x[i] = readlane32(ra.x[i], sl);
+    wb.x[i] = readlane32(rb.x[i], sl);
+  }
+  bbs_raw_move raw = bbs_move_without_jump(wa, wb, n_nodes, probs);
+  if (raw.move_type == BBS_MOVE_CHANGE_BETA)
+    raw.beta_jump = __uint_as_float(readlane32(__float_as_uint(jumps), sl));
+  return raw;
+}
+
+/* One step's proposal for the current tree (am, dm) and beta. */
+template <int NS>
+struct Proposal {
+  MoveCtx<NS> m;                  /* m.v, m.w after the swap of v and w    */
+  uint32_t nd_v, nd_w;            /* descendant counts of v, w            */
+  uint32_t parent_v, parent_w, desc_v, nondesc_v;
+  float nb;                       /* the reflected new beta               */
+  uint64_t pam[NS][NS], pdm[NS][NS];
+};
+
+/* Realisation (src/MutationTree.hpp:323-363) and the proposed rows
+ * (src/MutationTree.hpp:123-307). Without TRACE only what the drawn move
+ * type uses is computed (a beta move needs no tree data, prune and reattach
+ * a random non-descendant, a node swap the four rows, a subtree swap the
+ * parents and a random descendant); every value computed equals the full
+ * realisation's. */
+template <int NS, bool TRACE>
+__device__ __forceinline__ void propose(const bbs_raw_move &raw, uint32_t N, float cur_beta,
+                                        const uint64_t (&am)[NS][NS],
+                                        const uint64_t (&dm)[NS][NS], Proposal<NS> &P) {
+  MoveCtx<NS> &m = P.m;
+  m.type = raw.move_type;
+  m.related = false;
+  m.v_target = 0u;
+  m.w_target = 0u;
+  uint32_t v = raw.raw_v, w = raw.raw_w;
+  uint32_t nd_v = 0u, nd_w = 0u;
+  uint32_t parent_v = 0u, parent_w = 0u, desc_v = 0u, nondesc_v = 0u;
+  const bool subtrees = m.type == BBS_MOVE_SWAP_SUBTREES;
+  const bool prune = m.type == BBS_MOVE_PRUNE_REATTACH;
+  if (TRACE || m.type != BBS_MOVE_CHANGE_BETA) {
+    /* the four rows at once, then the swap of src/MutationTree.hpp:326-330
+     * (v an ancestor of w) as a selection */
+    uint64_t D0[NS], D1[NS], A0[NS], A1[NS];
+    fetch_row<NS>(am, v, D0);
+    fetch_row<NS>(am, w, D1);
+    fetch_row<NS>(dm, v, A0);
+    fetch_row<NS>(dm, w, A1);
+    const bool sw = row_bit<NS>(D0, w);
+    const uint32_t t = sw ? w : v;
+    w = sw ? v : w;
+    v = t;
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) {
+      m.Dv[wd] = sw ? D1[wd] : D0[wd];
+      m.Dw[wd] = sw ? D0[wd] : D1[wd];
+      m.Av[wd] = sw ? A1[wd] : A0[wd];
+      m.Aw[wd] = sw ? A0[wd] : A1[wd];
+      nd_v += (uint32_t)__popcll(m.Dv[wd]);
+    }
+  } else {
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) m.Dv[wd] = m.Dw[wd] = m.Av[wd] = m.Aw[wd] = 0ull;
+  }
+  m.v = v;
+  m.w = w;
+  if (TRACE || subtrees) {
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) nd_w += (uint32_t)__popcll(m.Dw[wd]);
+    m.related = row_bit<NS>(m.Dw, v);
+    parent_w = parent_of<NS>(dm, m.Aw, w, N);
+  }
+  if (TRACE || (subtrees && !m.related)) parent_v = parent_of<NS>(dm, m.Av, v, N);
+  if (TRACE || (subtrees && m.related)) {
+    const uint32_t i_d = (uint32_t)floorf(raw.raw_descendant_of_v * (float)nd_v);
+    desc_v = select_bit<NS>(m.Dv, i_d, N);
+  }
+  if (TRACE || prune) {
+    const uint32_t nn_v = N - nd_v;
+    const uint32_t i_n = (uint32_t)floorf(raw.raw_nondescendant_of_v * (float)nn_v);
+    uint64_t ndv[NS];
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) ndv[wd] = ~m.Dv[wd];
+    nondesc_v = select_bit<NS>(ndv, i_n, N);
+  }
+  float nb = cur_beta + raw.beta_jump;
+  if (nb < 0.0f) nb = fabsf(nb);
+  if (nb > 1.0f) nb = nb - 2.0f * (nb - 1.0f);
+
+  if (subtrees) {
+    m.v_target = parent_w;
+    m.w_target = m.related ? desc_v : parent_v;
+  } else if (prune) {
+    m.v_target = nondesc_v;
+  }
+  if (subtrees || prune) fetch_row<NS>(dm, m.v_target, m.At_v);
+  if (subtrees) fetch_row<NS>(dm, m.w_target, m.At_w);
+
+  apply_move<NS>(m, am, dm, P.pam, P.pdm);
+  P.nd_v = nd_v;
+  P.nd_w = nd_w;
+  P.parent_v = parent_v;
+  P.parent_w = parent_w;
+  P.desc_v = desc_v;
+  P.nondesc_v = nondesc_v;
+  P.nb = nb;
+}
+
+/* The rows of this lane's nodes (n_nodes of them) to rows[2 x][2]. */
+template <int NS>
+__device__ __forceinline__ void write_rows(uint64_t *am_out, uint64_t *dm_out,
+                                           const uint64_t (&am)[NS][NS],
+                                           const uint64_t (&dm)[NS][NS], uint32_t N) {
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane_id() + 64u * (uint32_t)k;
+    if (x < N) {
+#pragma unroll
+      for (int wd = 0; wd < NS; wd++) {
+        am_out[2u * x + (uint32_t)wd] = am[k][wd];
+        dm_out[2u * x + (uint32_t)wd] = dm[k][wd];
+      }
+      if (NS == 1) {
+        am_out[2u * x + 1u] = 0ull;
+        dm_out[2u * x + 1u] = 0ull;
+      }
+    }
+  }
+}
+
+/* The Hastings correction of src/Application.hpp:443-450: nd(v)/nd(w) for a
+ * subtree swap with w an ancestor of v (after the swap), else 1. */
+template <int NS>
+__device__ __forceinline__ float acceptance_corr(const Proposal<NS> &P) {
+  return (P.m.type == BBS_MOVE_SWAP_SUBTREES && P.m.related) ? (float)P.nd_v / (float)P.nd_w
+                                                             : 1.0f;
+}
+
+template <int NS>
+__device__ __forceinline__ bbs_step_trace step_trace(const Proposal<NS> &P, float prop_score,
+                                                     bool accept, bool new_max) {
+  bbs_step_trace tr;
+  tr.move_type = P.m.type;
+  tr.v = P.m.v;
+  tr.w = P.m.w;
+  tr.parent_of_v = P.parent_v;
+  tr.parent_of_w = P.parent_w;
+  tr.descendant_of_v = P.desc_v;
+  tr.nondescendant_of_v = P.nondesc_v;
+  tr.new_beta = P.nb;
+  tr.proposed_score = prop_score;
+  tr.flags = (accept ? 1u : 0u) | (new_max ? 2u : 0u);
+  return tr;
+}
+
 /* ------------------------------------------------------------ chain body */
 
 /* Everything one wavefront needs to advance one chain; all pointers point
@@ -264,91 +428,13 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
       jumps = bbs_beta_jump_at(rb, a.probs, bbs_coef_base());
     }
-    bbs_u32x4 wa, wb;
-    const uint32_t sl = step - rbase;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      wa.x[i] = readlane32(ra.x[i], sl);
-      wb.x[i] = readlane32(rb.x[i], sl);
-    }
-    bbs_raw_move raw = bbs_move_without_jump(wa, wb, N, a.probs);
-    if (raw.move_type == BBS_MOVE_CHANGE_BETA)
-      raw.beta_jump = __uint_as_float(readlane32(__float_as_uint(jumps), sl));
-    /* Realisation (src/MutationTree.hpp:323-363). Without a trace only what
-     * the drawn move type uses is computed (a beta move needs no tree data,
-     * prune and reattach a random non-descendant, a node swap the four rows,
-     * a subtree swap the parents and a random descendant); every value
-     * computed equals the full realisation's. */
-    MoveCtx<NS> m;
-    m.type = raw.move_type;
-    m.related = false;
-    m.v_target = 0u;
-    m.w_target = 0u;
-    uint32_t v = raw.raw_v, w = raw.raw_w;
-    uint32_t nd_v = 0u, nd_w = 0u;
-    uint32_t parent_v = 0u, parent_w = 0u, desc_v = 0u, nondesc_v = 0u;
-    const bool subtrees = m.type == BBS_MOVE_SWAP_SUBTREES;
-    const bool prune = m.type == BBS_MOVE_PRUNE_REATTACH;
-    if (TRACE || m.type != BBS_MOVE_CHANGE_BETA) {
-      /* the four rows at once, then the swap of src/MutationTree.hpp:326-330
-       * (v an ancestor of w) as a selection */
-      uint64_t D0[NS], D1[NS], A0[NS], A1[NS];
-      fetch_row<NS>(am, v, D0);
-      fetch_row<NS>(am, w, D1);
-      fetch_row<NS>(dm, v, A0);
-      fetch_row<NS>(dm, w, A1);
-      const bool sw = row_bit<NS>(D0, w);
-      const uint32_t t = sw ? w : v;
-      w = sw ? v : w;
-      v = t;
-#pragma unroll
-      for (int wd = 0; wd < NS; wd++) {
-        m.Dv[wd] = sw ? D1[wd] : D0[wd];
-        m.Dw[wd] = sw ? D0[wd] : D1[wd];
-        m.Av[wd] = sw ? A1[wd] : A0[wd];
-        m.Aw[wd] = sw ? A0[wd] : A1[wd];
-        nd_v += (uint32_t)__popcll(m.Dv[wd]);
-      }
-    } else {
-#pragma unroll
-      for (int wd = 0; wd < NS; wd++) m.Dv[wd] = m.Dw[wd] = m.Av[wd] = m.Aw[wd] = 0ull;
-    }
-    m.v = v;
-    m.w = w;
-    if (TRACE || subtrees) {
-#pragma unroll
-      for (int wd = 0; wd < NS; wd++) nd_w += (uint32_t)__popcll(m.Dw[wd]);
-      m.related = row_bit<NS>(m.Dw, v);
-      parent_w = parent_of<NS>(dm, m.Aw, w, N);
-    }
-    if (TRACE || (subtrees && !m.related)) parent_v = parent_of<NS>(dm, m.Av, v, N);
-    if (TRACE || (subtrees && m.related)) {
-      const uint32_t i_d = (uint32_t)floorf(raw.raw_descendant_of_v * (float)nd_v);
-      desc_v = select_bit<NS>(m.Dv, i_d, N);
-    }
-    if (TRACE || prune) {
-      const uint32_t nn_v = N - nd_v;
-      const uint32_t i_n = (uint32_t)floorf(raw.raw_nondescendant_of_v * (float)nn_v);
-      uint64_t ndv[NS];
-#pragma unroll
-      for (int wd = 0; wd < NS; wd++) ndv[wd] = ~m.Dv[wd];
-      nondesc_v = select_bit<NS>(ndv, i_n, N);
-    }
-    float nb = cur_beta + raw.beta_jump;
-    if (nb < 0.0f) nb = fabsf(nb);
-    if (nb > 1.0f) nb = nb - 2.0f * (nb - 1.0f);
-
-    if (subtrees) {
-      m.v_target = parent_w;
-      m.w_target = m.related ? desc_v : parent_v;
-    } else if (prune) {
-      m.v_target = nondesc_v;
-    }
-    if (subtrees || prune) fetch_row<NS>(dm, m.v_target, m.At_v);
-    if (subtrees) fetch_row<NS>(dm, m.w_target, m.At_w);
-
-    uint64_t pam[NS][NS], pdm[NS][NS];
-    apply_move<NS>(m, am, dm, pam, pdm);
+    const bbs_raw_move raw = draw_move(ra, rb, jumps, step - rbase, N, a.probs);
+    Proposal<NS> P;
+    propose<NS, TRACE>(raw, N, cur_beta, am, dm, P);
+    const MoveCtx<NS> &m = P.m;
+    const float nb = P.nb;
+    const uint64_t(&pam)[NS][NS] = P.pam;
+    const uint64_t(&pdm)[NS][NS] = P.pdm;
 
     float prop_beta = cur_beta;
     LogProbs Lp = L;
@@ -391,47 +477,18 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
     }
     const float prop_score = prop_raw + prop_prior;
 
-    float corr = 1.0f;
-    if (m.type == BBS_MOVE_SWAP_SUBTREES && m.related)
-      corr = (float)nd_v / (float)nd_w;
-    const float acc_prob = corr * expf_lanes((prop_score - cur_score) * a.gamma, exp2t);
+    const float acc_prob = acceptance_corr(P) * expf_lanes((prop_score - cur_score) * a.gamma, exp2t);
     const bool accept = acc_prob > raw.acceptance_level;
     const bool new_max = (a.is_first_chain && step == 0u) || prop_score > best_score;
 
-    if (TRACE && a.trace != nullptr && lane == 0u && leader) {
-      bbs_step_trace tr;
-      tr.move_type = m.type;
-      tr.v = v;
-      tr.w = w;
-      tr.parent_of_v = parent_v;
-      tr.parent_of_w = parent_w;
-      tr.descendant_of_v = desc_v;
-      tr.nondescendant_of_v = nondesc_v;
-      tr.new_beta = nb;
-      tr.proposed_score = prop_score;
-      tr.flags = (accept ? 1u : 0u) | (new_max ? 2u : 0u);
-      a.trace[step] = tr;
-    }
+    if (TRACE && a.trace != nullptr && lane == 0u && leader)
+      a.trace[step] = step_trace(P, prop_score, accept, new_max);
 
     if (new_max) {
       best_score = prop_score;
       best_beta = prop_beta;
       best_step = step;
-#pragma unroll
-      for (int k = 0; k < NS; k++) {
-        const uint32_t x = lane + 64u * (uint32_t)k;
-        if (x < N && leader) {
-#pragma unroll
-          for (int wd = 0; wd < NS; wd++) {
-            a.best_am[2u * x + (uint32_t)wd] = pam[k][wd];
-            a.best_dm[2u * x + (uint32_t)wd] = pdm[k][wd];
-          }
-          if (NS == 1) {
-            a.best_am[2u * x + 1u] = 0ull;
-            a.best_dm[2u * x + 1u] = 0ull;
-          }
-        }
-      }
+      if (leader) write_rows<NS>(a.best_am, a.best_dm, pam, pdm, N);
     }
     if (INCR && m.type == BBS_MOVE_CHANGE_BETA && !accept) {
       /* the table holds the rejected beta's scores: back to the current
@@ -468,21 +525,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
   }
 
   if (!leader) return;
-#pragma unroll
-  for (int k = 0; k < NS; k++) {
-    const uint32_t x = lane + 64u * (uint32_t)k;
-    if (x < N) {
-#pragma unroll
-      for (int wd = 0; wd < NS; wd++) {
-        a.am[2u * x + (uint32_t)wd] = am[k][wd];
-        a.dm[2u * x + (uint32_t)wd] = dm[k][wd];
-      }
-      if (NS == 1) {
-        a.am[2u * x + 1u] = 0ull;
-        a.dm[2u * x + 1u] = 0ull;
-      }
-    }
-  }
+  write_rows<NS>(a.am, a.dm, am, dm, N);
   if (lane == 0u) {
     *a.beta = cur_beta;
     *a.score = cur_score;
@@ -564,6 +607,276 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
     WaveLds lds = store.view();
     run_chain<NS, CS, W, TRACE, false, false>(v, lds, TableView{nullptr, 0u, 0u});
   }
+}
+
+/* ----------------------------------------------------- speculative chain */
+
+/* One speculative proposal's result, exchanged between the waves of a
+ * speculative chain. */
+struct SpecSlot {
+  float score, beta, L01, L11, prior;
+  uint32_t accept;
+  uint32_t pad[2];
+};
+
+/* Dynamic LDS of a speculative chain of k waves, in floats: the pair-score
+ * table of the current tree (shared, as ChainLds's), the exchange slots
+ * (2 rounds x k), the rows of an accepted proposal (n_nodes x (am, dm)),
+ * then per wave its proposal's tree rows, per-cell sums and penalties.
+ * The scan's unused reads past the table's last row stay inside the slots
+ * and rows that follow it; each wave's penalties get scan_reach floats. */
+struct SpecLds {
+  uint32_t table, slots, xrows, wave0, per_wave, sums_off, pen_off, total;
+  __host__ __device__ static SpecLds of(uint32_t n_nodes, uint32_t n_cells, int cs, uint32_t k) {
+    const uint32_t cslots = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
+    const uint32_t pen = TableView::pen_len(n_nodes) > TableView::scan_reach(n_nodes)
+                             ? TableView::pen_len(n_nodes)
+                             : TableView::scan_reach(n_nodes);
+    SpecLds l;
+    l.table = 0u;
+    l.slots = (TableView::floats_for(n_nodes, n_cells, cslots) + 3u) & ~3u;
+    l.xrows = l.slots + 2u * k * (uint32_t)(sizeof(SpecSlot) / 4u);
+    l.wave0 = l.xrows + 8u * n_nodes; /* 2 rows of 2 u64 per node */
+    l.sums_off = 4u * n_nodes;
+    l.pen_off = l.sums_off + cslots;
+    l.per_wave = (l.pen_off + pen + 3u) & ~3u;
+    l.total = l.wave0 + k * l.per_wave;
+    return l;
+  }
+};
+
+/*
+ * A chain on k waves that evaluate consecutive steps at once, for launches
+ * with SIMDs to spare (few chains). Each round, wave i proposes step s + i
+ * against the same current tree, with the shared pair-score table. That is
+ * the sequential chain's proposal of step s + i whenever steps s .. s + i - 1
+ * are all rejected, and most steps are. After one barrier every wave reads
+ * the k accept decisions. Steps up to and including the first accepted one
+ * are the sequential chain's; the rest are dropped and proposed again from
+ * the new tree. The accepting wave writes its tree to the exchange rows and
+ * updates the table (changed nodes, or every pair for a new beta); after a
+ * second barrier all waves take the new tree. Trajectories, traces and
+ * bests equal the sequential chain's step for step; only wave 0 writes the
+ * final state.
+ */
+template <int NS, int CS, int W, bool TRACE>
+__device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, const SpecLds &l,
+                                               uint32_t wave, uint32_t K) {
+  const uint32_t lane = lane_id();
+  const uint32_t N = a.n_nodes;
+  float *wbase = dyn + l.wave0 + wave * l.per_wave;
+  WaveLds lds{reinterpret_cast<uint4 *>(wbase), wbase + l.sums_off, wbase + l.pen_off};
+  const TableView tab{dyn + l.table, TableView::stride_for(N), a.n_cells};
+  SpecSlot *slots = reinterpret_cast<SpecSlot *>(dyn + l.slots);
+  uint64_t *xam = reinterpret_cast<uint64_t *>(dyn + l.xrows);
+  uint64_t *xdm = xam + 2u * N;
+
+  Cells<CS, W> cells;
+  load_cells<CS, W>(a.cells, a.n_cells, cells);
+  uint64_t am[NS][NS], dm[NS][NS];
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const uint32_t x = lane + 64u * (uint32_t)k;
+#pragma unroll
+    for (int wd = 0; wd < NS; wd++) {
+      am[k][wd] = x < N ? a.am[2u * x + (uint32_t)wd] : 0ull;
+      dm[k][wd] = x < N ? a.dm[2u * x + (uint32_t)wd] : 0ull;
+    }
+  }
+  float cur_beta = *a.beta;
+  LogProbs L;
+  L.L00 = a.L00;
+  L.L10 = a.L10;
+  L.L01 = bbs_logf(cur_beta);
+  L.L11 = bbs_logf_of_double(1.0 - (double)cur_beta);
+  float cur_prior = logscore_beta(a.prior, cur_beta);
+  uint32_t unused01[2], unused11[2];
+  /* wave 0 builds the shared table; the others score without writing it */
+  float t;
+  if (wave == 0u)
+    t = score_tree<NS, CS, W, false, true>(dm, cells, N, a.n_cells, L, lds, unused01, unused11,
+                                           tab);
+  else
+    t = score_tree<NS, CS, W, false, false>(dm, cells, N, a.n_cells, L, lds, unused01,
+                                            unused11);
+  float cur_score = a.score_init ? t + cur_prior : *a.score;
+  __syncthreads();
+
+  float best_score = *a.best_score;
+  float best_beta = *a.best_beta;
+  uint32_t best_step = *a.best_step;
+  const Exp2Lanes exp2t = exp2_lanes();
+  /* each wave's Philox blocks cover 64 steps from its rbase, one per lane */
+  uint32_t rbase = a.step_begin + wave;
+  bbs_u32x4 ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
+  bbs_u32x4 rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+  float jumps = bbs_beta_jump(rb, a.probs);
+  uint32_t round = 0u;
+  for (uint32_t s = a.step_begin; s < a.step_end; round++) {
+    const uint32_t my = s + wave; /* past step_end: proposed, never used */
+    if (my - rbase >= 64u) {
+      rbase = my;
+      ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
+      rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+      jumps = bbs_beta_jump_at(rb, a.probs, bbs_coef_base());
+    }
+    const bbs_raw_move raw = draw_move(ra, rb, jumps, my - rbase, N, a.probs);
+    Proposal<NS> P;
+    propose<NS, TRACE>(raw, N, cur_beta, am, dm, P);
+    const bool beta_move = P.m.type == BBS_MOVE_CHANGE_BETA;
+    float prop_beta = cur_beta;
+    LogProbs Lp = L;
+    float prop_prior = cur_prior;
+    if (beta_move) {
+      prop_beta = P.nb;
+      const uint32_t kb = bbs_coef_base();
+      Lp.L01 = bbs_logf_at(P.nb, kb);
+      Lp.L11 = bbs_logf_of_double_at(1.0 - (double)P.nb, kb);
+      prop_prior = logscore_beta(a.prior, P.nb, kb);
+    }
+    /* scored against the current tree's table, which no wave writes now */
+    uint64_t changed[NS];
+    float prop_raw;
+    if (!beta_move) {
+      bool diff[NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        diff[k] = false;
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) diff[k] |= P.pdm[k][wd] != dm[k][wd];
+        changed[k] = ballot(diff[k]);
+      }
+      store_pen<NS>(diff, N, lds);
+      store_rows<NS>(P.pdm, N, lds);
+      prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
+    } else {
+      prop_raw = score_tree<NS, CS, W, false, false>(P.pdm, cells, N, a.n_cells, Lp, lds,
+                                                     unused01, unused11);
+    }
+    const float prop_score = prop_raw + prop_prior;
+    const float acc_prob =
+        acceptance_corr(P) * expf_lanes((prop_score - cur_score) * a.gamma, exp2t);
+    const bool accept = acc_prob > raw.acceptance_level;
+
+    SpecSlot *sr = slots + (round & 1u) * K; /* a round's slots are rewritten two rounds on */
+    if (lane == 0u) {
+      SpecSlot o;
+      o.score = prop_score;
+      o.beta = prop_beta;
+      o.L01 = Lp.L01;
+      o.L11 = Lp.L11;
+      o.prior = prop_prior;
+      o.accept = accept ? 1u : 0u;
+      o.pad[0] = o.pad[1] = 0u;
+      sr[wave] = o;
+    }
+    __syncthreads();
+    const uint32_t n_act = a.step_end - s < K ? a.step_end - s : K;
+    uint32_t j = n_act; /* the first accepted step of the round, if any */
+    for (uint32_t i = 0; i < n_act; i++)
+      if (sr[i].accept != 0u) {
+        j = i;
+        break;
+      }
+    const uint32_t committed = j < n_act ? j + 1u : n_act;
+    /* best tracking over the committed proposals in step order
+     * (src/Application.hpp:456) */
+    uint32_t last_max = K;
+    bool my_max = false;
+    for (uint32_t i = 0; i < committed; i++) {
+      const float ps = sr[i].score;
+      if ((a.is_first_chain && s + i == 0u) || ps > best_score) {
+        best_score = ps;
+        best_beta = sr[i].beta;
+        best_step = s + i;
+        last_max = i;
+        if (i == wave) my_max = true;
+      }
+    }
+    if (TRACE && a.trace != nullptr && lane == 0u && wave < committed)
+      a.trace[my] = step_trace(P, prop_score, wave == j, my_max);
+    if (wave == last_max) write_rows<NS>(a.best_am, a.best_dm, P.pam, P.pdm, N);
+    if (j < n_act) {
+      /* the accepting wave publishes its tree and brings the table to it:
+       * the changed nodes, or every pair for a new beta (its rows are the
+       * tree's, in its LDS) */
+      if (wave == j) {
+        write_rows<NS>(xam, xdm, P.pam, P.pdm, N);
+        float R[2];
+        bool n0[2];
+        if (!beta_move)
+          changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
+        else
+          node_maxima<NS, CS, W, false, true>(cells, N, Lp, lds, tab, R, n0, unused01,
+                                               unused11);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        const uint32_t x = lane + 64u * (uint32_t)k;
+#pragma unroll
+        for (int wd = 0; wd < NS; wd++) {
+          am[k][wd] = x < N ? xam[2u * x + (uint32_t)wd] : 0ull;
+          dm[k][wd] = x < N ? xdm[2u * x + (uint32_t)wd] : 0ull;
+        }
+      }
+      const SpecSlot &o = sr[j];
+      cur_score = o.score;
+      cur_beta = o.beta;
+      L.L01 = o.L01;
+      L.L11 = o.L11;
+      cur_prior = o.prior;
+    }
+    s += committed;
+  }
+  if (wave != 0u) return;
+  write_rows<NS>(a.am, a.dm, am, dm, N);
+  if (lane == 0u) {
+    *a.beta = cur_beta;
+    *a.score = cur_score;
+    *a.best_score = best_score;
+    *a.best_beta = best_beta;
+    *a.best_step = best_step;
+  }
+}
+
+/* One chain per workgroup of k = blockDim / 64 waves running it
+ * speculatively (run_chain_spec). */
+template <int NS, int CS, int W, bool TRACE>
+__global__ __launch_bounds__(512) void chain_spec_kernel(ChainArgs a) {
+  extern __shared__ __align__(16) float dyn[];
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
+  const uint32_t K = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  const uint64_t rows = 2ull * chain * a.n_nodes;
+  ChainView v;
+  v.cells = a.cells;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + chain;
+  v.score = a.score + chain;
+  v.best_beta = a.best_beta + chain;
+  v.best_score = a.best_score + chain;
+  v.best_step = a.best_step + chain;
+  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
+                                          : nullptr;
+  v.seed = a.seed;
+  v.job = a.job;
+  v.chain = a.chain_offset + chain;
+  v.step_begin = a.step_begin;
+  v.step_end = a.step_end;
+  v.n_cells = a.n_cells;
+  v.n_nodes = a.n_nodes;
+  v.is_first_chain = a.chain_offset + chain == 0u;
+  v.score_init = a.score_init != 0u;
+  v.L00 = a.L00;
+  v.L10 = a.L10;
+  v.gamma = a.gamma;
+  v.prior = a.prior;
+  v.probs = a.probs;
+  run_chain_spec<NS, CS, W, TRACE>(v, dyn, SpecLds::of(a.n_nodes, a.n_cells, CS, K), wave, K);
 }
 
 /* One chain per workgroup of SPLIT waves, split by cells: wave i scores
@@ -1267,6 +1580,38 @@ hipError_t launch_chain_split(const ChainArgs &a, bool trace, uint32_t split,
   if (ns == NS_ && w == W_) return launch_chain_split_shape<NS_, W_>(a, trace, split, bytes, stream);
   X2(1, 1) X2(1, 2) X2(2, 2) X2(2, 3) X2(2, 4)
 #undef X2
+  return hipErrorInvalidValue;
+}
+
+uint32_t chain_spec_bytes(uint32_t n_nodes, uint32_t n_cells, uint32_t k) {
+  if (k != 2u && k != 4u && k != 8u) return 0u;
+  int ns, cs, w;
+  shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
+  const uint32_t bytes = 4u * SpecLds::of(n_nodes, n_cells, cs, k).total;
+  return bytes <= kMaxGroupLds ? bytes : 0u;
+}
+
+template <int NS, int CS, int W>
+static hipError_t launch_chain_spec_shape(const ChainArgs &a, bool trace, uint32_t k,
+                                          uint32_t bytes, hipStream_t stream) {
+  const dim3 grid(a.n_chains), block(64u * k);
+  if (trace)
+    hipLaunchKernelGGL((chain_spec_kernel<NS, CS, W, true>), grid, block, bytes, stream, a);
+  else
+    hipLaunchKernelGGL((chain_spec_kernel<NS, CS, W, false>), grid, block, bytes, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain_spec(const ChainArgs &a, bool trace, uint32_t k, hipStream_t stream) {
+  const uint32_t bytes = chain_spec_bytes(a.n_nodes, a.n_cells, k);
+  if (bytes == 0u) return hipErrorInvalidValue;
+  int ns, cs, w;
+  shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
+#define XP(NS_, CS_, W_)                                                       \
+  if (ns == NS_ && cs == CS_ && w == W_)                                       \
+    return launch_chain_spec_shape<NS_, CS_, W_>(a, trace, k, bytes, stream);
+  BBS_SHAPES(XP)
+#undef XP
   return hipErrorInvalidValue;
 }
 

@@ -474,28 +474,31 @@ def plain_64x63_bench(with_cpu):
 
 
 def few_chains_bench():
-    """8 chains of a wide dataset (the plain launch with fewer chains than
-    SIMDs, as the command line runs it): one wave per chain against each chain
-    split over two waves by cells (the automatic choice above 64 cells)."""
+    """8 chains (the plain launch with fewer chains than SIMDs, as the
+    command line runs it): one wave per chain, against 8 waves per chain
+    proposing consecutive steps at once (speculation) and against each chain's
+    cells split over two waves; the automatic choice is speculation up to 96
+    cells and the split above."""
     out = {}
-    for G, C, steps in ((94, 95, 50_000), (127, 128, 30_000)):
+    modes = (("one_wave", 1, 1), ("speculation_8", 8, 1), ("split_2", 1, 2))
+    for G, C, steps in ((63, 64, 100_000), (94, 95, 50_000), (127, 128, 30_000)):
         m, _, _ = B.generate_dataset(2, G, C, 1e-4, 0.3, 0.15)
         mut, kno = B.encode_matrix(m)
         p = B.default_params(n_chains=8, n_steps=steps, seed=2, alpha_mean=1e-4, beta_mean=0.3,
                              beta_sd=0.05, prob_beta_change=0.1)
         row = {"chains": 8, "steps": steps}
         best = {}
-        for k in (1, 2):
+        for name, spec, split in modes:
             with B.Context(p, mut, kno, G) as ctx:
-                ctx.set_chain_split(k)
+                ctx.set_speculation(spec)
+                ctx.set_chain_split(split)
                 ms = ctx.run()
-                best[k] = ctx.best()[3]
-            row[f"{k}_wave_per_chain_steps_per_s" if k == 1 else f"{k}_waves_per_chain_steps_per_s"] = \
-                8 * steps / (ms / 1e3)
-        row["best_ll_identical"] = bool(np.float32(best[1]).tobytes() == np.float32(best[2]).tobytes())
+                best[name] = np.float32(ctx.best()[3]).tobytes()
+            row[f"{name}_steps_per_s"] = 8 * steps / (ms / 1e3)
+        row["best_ll_identical"] = len(set(best.values())) == 1
         out[f"{G}x{C}"] = row
-        log(f"[bench] {G}x{C} 8 chains: {row['1_wave_per_chain_steps_per_s']:.3e} steps/s one wave "
-            f"per chain, {row['2_waves_per_chain_steps_per_s']:.3e} split over two")
+        log(f"[bench] {G}x{C} 8 chains: " + ", ".join(
+            f"{n} {row[n + '_steps_per_s']:.3e}" for n, _, _ in modes) + " steps/s")
     return out
 
 

@@ -136,6 +136,25 @@ int bbs_set_scoring_mode(bbs_ctx *ctx, int mode);
  * wave without cells). */
 int bbs_set_chain_split(bbs_ctx *ctx, uint32_t waves_per_chain);
 
+/* Speculative waves per chain of the plain launch (no reference counterpart:
+ * the reference evaluates one step of a chain at a time,
+ * src/Application.hpp:412-487). k = 2, 4 or 8: each chain runs on k
+ * wavefronts; in each round wave i proposes step s + i against the same
+ * current tree and pair-score table, which is the chain's own proposal of
+ * that step whenever steps s .. s + i - 1 are rejected. The steps up to the
+ * first accepted one are kept and the rest proposed again from the new tree,
+ * so a round advances the chain by up to k steps in about the time of one.
+ * A round lasts as long as its slowest proposal (a beta move rescores every
+ * pair), so the gain is below k. 1: off. 0 (the default): 8 or 4 when every
+ * chain's waves fit the device's SIMDs at once, the incremental scorer is in
+ * use and the dataset has at most 96 cells (over 96 the cell split of
+ * bbs_set_chain_split gains more); when it applies it takes precedence over
+ * bbs_set_chain_split. Trajectories, traces and
+ * bests are bit-identical for every value; with BBS_SCORING_FULL it is off.
+ * BBS_ERR_ARG for other values, BBS_ERR_SIZE if the k waves' LDS does not fit
+ * one workgroup. */
+int bbs_set_speculation(bbs_ctx *ctx, uint32_t waves_per_chain);
+
 /* Replaces Application::run_simulation (src/Application.hpp:197-219): runs
  * all chains for n_steps steps and blocks; *makespan_ms (nullable) receives
  * the device time in milliseconds, measured with HIP events. */

@@ -164,14 +164,19 @@ CHAIN_CASES = [
 ]
 
 
+MODES = ["full", "incremental", "split2", "split4", "spec2", "spec4", "spec8"]
+
+
 def _set_mode(ctx, mode):
     """full: every pair rescored; incremental: the pair-score table, one
     wave per chain; split2 / split4: the table with each chain split over 2
-    or 4 waves by cells."""
+    or 4 waves by cells; spec2 / spec4 / spec8: the table with 2, 4 or 8
+    waves per chain proposing consecutive steps at once."""
     try:
         ctx.set_scoring_mode("full" if mode == "full" else "incremental")
         if mode != "full":
-            ctx.set_chain_split({"incremental": 1, "split2": 2, "split4": 4}[mode])
+            ctx.set_speculation(int(mode[4:]) if mode.startswith("spec") else 1)
+            ctx.set_chain_split(int(mode[5:]) if mode.startswith("split") else 1)
     except B.BbsError as e:
         if mode != "full" and e.code == -2:
             pytest.skip(f"no {mode} scorer for this size (LDS of a workgroup, or a wave "
@@ -179,7 +184,7 @@ def _set_mode(ctx, mode):
         raise
 
 
-@pytest.mark.parametrize("mode", ["full", "incremental", "split2", "split4"])
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("case", CHAIN_CASES, ids=lambda c: f"G{c[0]}C{c[1]}")
 def test_chain_trace_replay(case, mode):
     G, C, seed, R, S, pb, bm, bsd = case
@@ -238,7 +243,7 @@ def test_incremental_equals_full_over_long_runs(G, C):
     say(f"[incremental] G={G} C={C}")
     p, mut, kno = _chain_case(G, C, 12, 16, 4000, 0.1, beta_mean=0.3, beta_sd=0.05)
     out = {}
-    for mode in ("full", "incremental", "split2", "split4"):
+    for mode in MODES:
         with B.Context(p, mut, kno, G) as ctx:
             try:
                 _set_mode(ctx, mode)
@@ -370,6 +375,10 @@ def test_errors_are_reported():
             ctx.set_chain_split(3)
         assert e.value.code == B.ERR_ARG
         ctx.set_chain_split(4)  # 4 cells, one per wave
+        with pytest.raises(B.BbsError) as e:
+            ctx.set_speculation(3)
+        assert e.value.code == B.ERR_ARG
+        ctx.set_speculation(8)
     m3 = _iid_matrix(2, 4, 3)
     mut3, kno3 = B.encode_matrix(m3)
     with B.Context(B.default_params(n_chains=2, n_steps=10), mut3, kno3, 4) as ctx:
