@@ -619,14 +619,15 @@ struct SpecSlot {
   uint32_t pad[2];
 };
 
-/* Dynamic LDS of a speculative chain of k waves, in floats: the pair-score
- * table of the current tree (shared, as ChainLds's), the exchange slots
- * (2 rounds x k), the rows of an accepted proposal (n_nodes x (am, dm)),
- * then per wave its proposal's tree rows, per-cell sums and penalties.
- * The scan's unused reads past the table's last row stay inside the slots
- * and rows that follow it; each wave's penalties get scan_reach floats. */
+/* Dynamic LDS of a speculative chain of k waves, in floats: two pair-score
+ * tables (as ChainLds's: the current tree's, and one a beta proposal writes
+ * its scores to), the exchange slots (2 rounds x k), the rows of an
+ * accepted proposal (n_nodes x (am, dm)), then per wave its proposal's tree
+ * rows, per-cell sums and penalties. The scan's unused reads past a table's
+ * last row stay inside what follows it; each wave's penalties get
+ * scan_reach floats. */
 struct SpecLds {
-  uint32_t table, slots, xrows, wave0, per_wave, sums_off, pen_off, total;
+  uint32_t table, table_floats, slots, xrows, wave0, per_wave, sums_off, pen_off, total;
   __host__ __device__ static SpecLds of(uint32_t n_nodes, uint32_t n_cells, int cs, uint32_t k) {
     const uint32_t cslots = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
     const uint32_t pen = TableView::pen_len(n_nodes) > TableView::scan_reach(n_nodes)
@@ -634,7 +635,8 @@ struct SpecLds {
                              : TableView::scan_reach(n_nodes);
     SpecLds l;
     l.table = 0u;
-    l.slots = (TableView::floats_for(n_nodes, n_cells, cslots) + 3u) & ~3u;
+    l.table_floats = (TableView::floats_for(n_nodes, n_cells, cslots) + 3u) & ~3u;
+    l.slots = 2u * l.table_floats;
     l.xrows = l.slots + 2u * k * (uint32_t)(sizeof(SpecSlot) / 4u);
     l.wave0 = l.xrows + 8u * n_nodes; /* 2 rows of 2 u64 per node */
     l.sums_off = 4u * n_nodes;
@@ -666,7 +668,11 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
   const uint32_t N = a.n_nodes;
   float *wbase = dyn + l.wave0 + wave * l.per_wave;
   WaveLds lds{reinterpret_cast<uint4 *>(wbase), wbase + l.sums_off, wbase + l.pen_off};
-  const TableView tab{dyn + l.table, TableView::stride_for(N), a.n_cells};
+  /* tabs[cur] holds the current tree's pair scores */
+  const TableView tabs[2] = {TableView{dyn + l.table, TableView::stride_for(N), a.n_cells},
+                             TableView{dyn + l.table + l.table_floats,
+                                       TableView::stride_for(N), a.n_cells}};
+  uint32_t cur = 0u;
   SpecSlot *slots = reinterpret_cast<SpecSlot *>(dyn + l.slots);
   uint64_t *xam = reinterpret_cast<uint64_t *>(dyn + l.xrows);
   uint64_t *xdm = xam + 2u * N;
@@ -695,7 +701,7 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
   float t;
   if (wave == 0u)
     t = score_tree<NS, CS, W, false, true>(dm, cells, N, a.n_cells, L, lds, unused01, unused11,
-                                           tab);
+                                           tabs[0]);
   else
     t = score_tree<NS, CS, W, false, false>(dm, cells, N, a.n_cells, L, lds, unused01,
                                             unused11);
@@ -720,10 +726,17 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
       rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
       jumps = bbs_beta_jump_at(rb, a.probs, bbs_coef_base());
     }
+    /* the round's beta moves (lane i: step s + i), from their move draws */
+    const uint64_t betas = ballot(
+        lane < K && bbs_move_type_of(bbs_u01f(bbs_philox_block(a.seed, a.job, a.chain, s + lane,
+                                                               BBS_PURPOSE_MOVE_A).x[0]),
+                                     a.probs) == BBS_MOVE_CHANGE_BETA);
+    const uint32_t first_beta = betas ? (uint32_t)__ffsll((long long)betas) - 1u : K;
     const bbs_raw_move raw = draw_move(ra, rb, jumps, my - rbase, N, a.probs);
     Proposal<NS> P;
     propose<NS, TRACE>(raw, N, cur_beta, am, dm, P);
     const bool beta_move = P.m.type == BBS_MOVE_CHANGE_BETA;
+    const TableView tab = tabs[cur];
     float prop_beta = cur_beta;
     LogProbs Lp = L;
     float prop_prior = cur_prior;
@@ -749,6 +762,11 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
       store_pen<NS>(diff, N, lds);
       store_rows<NS>(P.pdm, N, lds);
       prop_raw = score_changed<NS, CS, W>(changed, cells, N, a.n_cells, Lp, lds, tab);
+    } else if (wave == first_beta) {
+      /* the round's first beta move writes its pair scores to the spare
+       * table, which becomes the current one if this step is accepted */
+      prop_raw = score_tree<NS, CS, W, false, true>(P.pdm, cells, N, a.n_cells, Lp, lds,
+                                                    unused01, unused11, tabs[cur ^ 1u]);
     } else {
       prop_raw = score_tree<NS, CS, W, false, false>(P.pdm, cells, N, a.n_cells, Lp, lds,
                                                      unused01, unused11);
@@ -798,18 +816,21 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
     if (wave == last_max) write_rows<NS>(a.best_am, a.best_dm, P.pam, P.pdm, N);
     if (j < n_act) {
       /* the accepting wave publishes its tree and brings the table to it:
-       * the changed nodes, or every pair for a new beta (its rows are the
-       * tree's, in its LDS) */
+       * the changed nodes; for a new beta, the spare table already holds it
+       * when this is the round's first beta move, else every pair is
+       * rescored into the current table (its rows are the tree's, in its
+       * LDS) */
       if (wave == j) {
         write_rows<NS>(xam, xdm, P.pam, P.pdm, N);
         float R[2];
         bool n0[2];
         if (!beta_move)
           changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
-        else
+        else if (j != first_beta)
           node_maxima<NS, CS, W, false, true>(cells, N, Lp, lds, tab, R, n0, unused01,
                                                unused11);
       }
+      if (j == first_beta) cur ^= 1u;
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < NS; k++) {

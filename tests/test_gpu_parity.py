@@ -161,6 +161,9 @@ CHAIN_CASES = [
     (64, 65, 9, 2, 120, 0.1, 0.3, 0.1),
     (3, 70, 10, 3, 200, 0.1, 0.25, 0.1),
     (2, 1, 11, 2, 100, 0.1, 0.25, 0.1),
+    # half the moves change beta: rounds of the speculative chain with
+    # several beta proposals, accepted and rejected ones
+    (20, 70, 12, 3, 300, 0.5, 0.3, 0.1),
 ]
 
 
