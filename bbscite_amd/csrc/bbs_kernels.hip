@@ -668,10 +668,12 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
   const uint32_t N = a.n_nodes;
   float *wbase = dyn + l.wave0 + wave * l.per_wave;
   WaveLds lds{reinterpret_cast<uint4 *>(wbase), wbase + l.sums_off, wbase + l.pen_off};
-  /* tabs[cur] holds the current tree's pair scores */
-  const TableView tabs[2] = {TableView{dyn + l.table, TableView::stride_for(N), a.n_cells},
-                             TableView{dyn + l.table + l.table_floats,
-                                       TableView::stride_for(N), a.n_cells}};
+  /* table(cur) holds the current tree's pair scores (a base computed, not
+   * an indexed pair of views, which would live in scratch memory) */
+  const uint32_t stride = TableView::stride_for(N);
+  auto table = [&](uint32_t which) {
+    return TableView{dyn + l.table + (which ? l.table_floats : 0u), stride, a.n_cells};
+  };
   uint32_t cur = 0u;
   SpecSlot *slots = reinterpret_cast<SpecSlot *>(dyn + l.slots);
   uint64_t *xam = reinterpret_cast<uint64_t *>(dyn + l.xrows);
@@ -701,7 +703,7 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
   float t;
   if (wave == 0u)
     t = score_tree<NS, CS, W, false, true>(dm, cells, N, a.n_cells, L, lds, unused01, unused11,
-                                           tabs[0]);
+                                           table(0u));
   else
     t = score_tree<NS, CS, W, false, false>(dm, cells, N, a.n_cells, L, lds, unused01,
                                             unused11);
@@ -712,31 +714,44 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
   float best_beta = *a.best_beta;
   uint32_t best_step = *a.best_step;
   const Exp2Lanes exp2t = exp2_lanes();
-  /* each wave's Philox blocks cover 64 steps from its rbase, one per lane */
-  uint32_t rbase = a.step_begin + wave;
-  bbs_u32x4 ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
-  bbs_u32x4 rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
-  float jumps = bbs_beta_jump(rb, a.probs);
+  /* Every wave holds the same two Philox blocks, steps [rbase, rbase + 128),
+   * one step per lane in each, and which of those steps change beta (bit i
+   * of bm0 / bm1: step rbase + i / rbase + 64 + i). A round starts at
+   * s < rbase + 64, so its k steps are all there. */
+  uint32_t rbase = a.step_begin;
+  bbs_u32x4 ra0 = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
+  bbs_u32x4 rb0 = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
+  bbs_u32x4 ra1 = bbs_philox_block(a.seed, a.job, a.chain, rbase + 64u + lane, BBS_PURPOSE_MOVE_A);
+  bbs_u32x4 rb1 = bbs_philox_block(a.seed, a.job, a.chain, rbase + 64u + lane, BBS_PURPOSE_MOVE_B);
+  float jumps0 = bbs_beta_jump(rb0, a.probs), jumps1 = bbs_beta_jump(rb1, a.probs);
+  uint64_t bm0 = ballot(bbs_move_type_of(bbs_u01f(ra0.x[0]), a.probs) == BBS_MOVE_CHANGE_BETA);
+  uint64_t bm1 = ballot(bbs_move_type_of(bbs_u01f(ra1.x[0]), a.probs) == BBS_MOVE_CHANGE_BETA);
   uint32_t round = 0u;
   for (uint32_t s = a.step_begin; s < a.step_end; round++) {
     const uint32_t my = s + wave; /* past step_end: proposed, never used */
-    if (my - rbase >= 64u) {
-      rbase = my;
-      ra = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_A);
-      rb = bbs_philox_block(a.seed, a.job, a.chain, rbase + lane, BBS_PURPOSE_MOVE_B);
-      jumps = bbs_beta_jump_at(rb, a.probs, bbs_coef_base());
+    if (s - rbase >= 64u) {
+      rbase += 64u;
+      ra0 = ra1;
+      rb0 = rb1;
+      jumps0 = jumps1;
+      bm0 = bm1;
+      ra1 = bbs_philox_block(a.seed, a.job, a.chain, rbase + 64u + lane, BBS_PURPOSE_MOVE_A);
+      rb1 = bbs_philox_block(a.seed, a.job, a.chain, rbase + 64u + lane, BBS_PURPOSE_MOVE_B);
+      jumps1 = bbs_beta_jump_at(rb1, a.probs, bbs_coef_base());
+      bm1 = ballot(bbs_move_type_of(bbs_u01f(ra1.x[0]), a.probs) == BBS_MOVE_CHANGE_BETA);
     }
-    /* the round's beta moves (lane i: step s + i), from their move draws */
-    const uint64_t betas = ballot(
-        lane < K && bbs_move_type_of(bbs_u01f(bbs_philox_block(a.seed, a.job, a.chain, s + lane,
-                                                               BBS_PURPOSE_MOVE_A).x[0]),
-                                     a.probs) == BBS_MOVE_CHANGE_BETA);
+    /* the round's first beta move (bit i: step s + i) */
+    const uint32_t off = s - rbase;
+    const uint64_t betas = ((bm0 >> off) | (off ? bm1 << (64u - off) : 0ull)) &
+                           (K < 64u ? (1ull << K) - 1ull : ~0ull);
     const uint32_t first_beta = betas ? (uint32_t)__ffsll((long long)betas) - 1u : K;
-    const bbs_raw_move raw = draw_move(ra, rb, jumps, my - rbase, N, a.probs);
+    const uint32_t sl = my - rbase;
+    const bbs_raw_move raw = sl < 64u ? draw_move(ra0, rb0, jumps0, sl, N, a.probs)
+                                      : draw_move(ra1, rb1, jumps1, sl - 64u, N, a.probs);
     Proposal<NS> P;
     propose<NS, TRACE>(raw, N, cur_beta, am, dm, P);
     const bool beta_move = P.m.type == BBS_MOVE_CHANGE_BETA;
-    const TableView tab = tabs[cur];
+    const TableView tab = table(cur);
     float prop_beta = cur_beta;
     LogProbs Lp = L;
     float prop_prior = cur_prior;
@@ -766,7 +781,7 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
       /* the round's first beta move writes its pair scores to the spare
        * table, which becomes the current one if this step is accepted */
       prop_raw = score_tree<NS, CS, W, false, true>(P.pdm, cells, N, a.n_cells, Lp, lds,
-                                                    unused01, unused11, tabs[cur ^ 1u]);
+                                                    unused01, unused11, table(cur ^ 1u));
     } else {
       prop_raw = score_tree<NS, CS, W, false, false>(P.pdm, cells, N, a.n_cells, Lp, lds,
                                                      unused01, unused11);

@@ -545,7 +545,8 @@ static uint32_t chain_split(const bbs_ctx *ctx) {
    * hold two cell slots (over 64 cells): lanes already score cells in
    * parallel, so fewer cells per wave alone removes no instruction. Measured
    * with 8 chains: 63 x 64 loses 11 % with 2 waves, 94 x 95 gains 5 %,
-   * 127 x 128 17 %; 4 waves never beat 2. */
+   * 127 x 128 17 %; 4 waves never beat 2. Speculation gains more where it
+   * fits; the split serves launches of 257-512 chains over 64 cells. */
   const uint64_t simds = 4ull * (ctx->n_cus ? ctx->n_cus : 1u);
   if (ctx->n_cells > 64u && (uint64_t)ctx->params.n_chains * 2u <= simds &&
       bbs::chain_split_bytes(ctx->n_nodes, ctx->n_cells, 2u) != 0u)
@@ -555,14 +556,13 @@ static uint32_t chain_split(const bbs_ctx *ctx) {
 
 /* Speculative waves per chain of this context's launches (bbs_set_speculation).
  * Automatically 8 or 4 when every chain's waves fit the device's SIMDs at
- * once and the table is in use, up to 96 cells. Measured with 8 chains
- * against one wave per chain: 30 x 40 +81 % (8 waves), 94 x 95 +27 % (8),
- * 63 x 64 +14 % (4); 2 waves gain at most 3 %. Over 96 cells the cell split
- * serves better (127 x 128: speculation +9 %, split +17 %). */
+ * once and the table is in use. Measured with 8 chains against one wave per
+ * chain: 63 x 64 +21 %, 94 x 95 +51 % (8 waves), 127 x 128 +25 % (4 waves;
+ * the cell split gives +20 % there); 2 waves gain little. */
 static uint32_t chain_speculation(const bbs_ctx *ctx) {
   if (ctx->scoring_mode == BBS_SCORING_FULL) return 1u; /* speculation keeps a table */
   if (ctx->speculation != 0u) return ctx->speculation;
-  if (chain_table(ctx) == 0u || ctx->n_cells > 96u) return 1u;
+  if (chain_table(ctx) == 0u) return 1u;
   const uint64_t simds = 4ull * (ctx->n_cus ? ctx->n_cus : 1u);
   for (uint32_t k = 8u; k >= 4u; k /= 2u)
     if ((uint64_t)ctx->params.n_chains * k <= simds &&

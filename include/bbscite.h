@@ -127,8 +127,9 @@ int bbs_set_scoring_mode(bbs_ctx *ctx, int mode);
  * cell sum, with one workgroup barrier per step; with over 64 cells (two
  * cell slots in one wave) it shortens a step where the launch has fewer
  * chains than the device has SIMDs. 0 (the default): 2 when the dataset has
- * over 64 cells, 2 n_chains waves fit the device's SIMDs and the incremental
- * scorer is in use, else 1. Results are
+ * over 64 cells, 2 n_chains waves fit the device's SIMDs, the incremental
+ * scorer is in use and no speculation applies (bbs_set_speculation), else 1.
+ * Results are
  * bit-identical for every value; with BBS_SCORING_FULL a chain always has
  * one wavefront (a split chain keeps the incremental scorer's table).
  * BBS_ERR_ARG for other values, BBS_ERR_SIZE
@@ -146,10 +147,9 @@ int bbs_set_chain_split(bbs_ctx *ctx, uint32_t waves_per_chain);
  * so a round advances the chain by up to k steps in about the time of one.
  * A round lasts as long as its slowest proposal (a beta move rescores every
  * pair), so the gain is below k. 1: off. 0 (the default): 8 or 4 when every
- * chain's waves fit the device's SIMDs at once, the incremental scorer is in
- * use and the dataset has at most 96 cells (over 96 the cell split of
- * bbs_set_chain_split gains more); when it applies it takes precedence over
- * bbs_set_chain_split. Trajectories, traces and
+ * chain's waves fit the device's SIMDs at once and the incremental scorer is
+ * in use; when it applies it takes precedence over bbs_set_chain_split.
+ * Trajectories, traces and
  * bests are bit-identical for every value; with BBS_SCORING_FULL it is off.
  * BBS_ERR_ARG for other values, BBS_ERR_SIZE if the k waves' LDS does not fit
  * one workgroup. */
