@@ -164,6 +164,7 @@ _SIGS = {
     "bbs_create": (C.c_int, [C.POINTER(Params), _u32, _u32, _P, _P, C.POINTER(_P)]),
     "bbs_set_initial_states": (C.c_int, [_P, _P, _P, _P, _P]),
     "bbs_set_trace": (C.c_int, [_P, _u32]),
+    "bbs_set_proposal_trace": (C.c_int, [_P, _u32]),
     "bbs_set_scoring_mode": (C.c_int, [_P, C.c_int]),
     "bbs_set_chain_split": (C.c_int, [_P, C.c_uint32]),
     "bbs_set_speculation": (C.c_int, [_P, C.c_uint32]),
@@ -174,6 +175,7 @@ _SIGS = {
     "bbs_get_chain_results": (C.c_int, [_P, _P, _P, _P, _P]),
     "bbs_get_chain_states": (C.c_int, [_P, _P, _P]),
     "bbs_get_trace": (C.c_int, [_P, _P]),
+    "bbs_get_proposal_trace": (C.c_int, [_P, _P, _P]),
     "bbs_score_trees": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
     "bbs_score_trees_noalloc": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
     "bbs_bench_score_random": (C.c_int, [_P, _u64, _u32, _u32, _u32, _P, _P, _u32]),
@@ -441,6 +443,10 @@ class Context:
     def set_trace(self, n):
         self._c(lib().bbs_set_trace(self.h, n))
 
+    def set_proposal_trace(self, n):
+        """Record every proposed tree's rows for chains [0, n)."""
+        self._c(lib().bbs_set_proposal_trace(self.h, n))
+
     def set_scoring_mode(self, mode):
         """"auto", "full" or "incremental" (results are identical)."""
         code = {"auto": 0, "full": 1, "incremental": 2}[mode]
@@ -495,6 +501,15 @@ class Context:
         out = np.zeros((n_traced, self.params.n_steps), TRACE_DTYPE)
         self._c(lib().bbs_get_trace(self.h, _ptr(out)))
         return out
+
+    def proposal_trace(self, n_traced):
+        """(am, dm): [n_traced][n_steps][n_nodes][2] uint64 rows of every
+        step's proposed tree."""
+        shape = (n_traced, self.params.n_steps, self.n_genes + 1, 2)
+        am = np.zeros(shape, np.uint64)
+        dm = np.zeros(shape, np.uint64)
+        self._c(lib().bbs_get_proposal_trace(self.h, _ptr(am), _ptr(dm)))
+        return am, dm
 
     def score_trees(self, dm_rows, betas=None, with_prior=True, counts=False,
                     cell_lanes=False, valu_lanes=False, no_alloc=False):

@@ -320,6 +320,7 @@ struct ChainView {
   float *beta, *score, *best_beta, *best_score;
   uint32_t *best_step;
   bbs_step_trace *trace; /* [step] or null                        */
+  uint64_t *prop_am, *prop_dm; /* [step][n_nodes][2] or null: proposals */
   uint64_t seed;
   uint32_t job, chain; /* Philox counter words                          */
   uint32_t step_begin, step_end;
@@ -483,6 +484,8 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
 
     if (TRACE && a.trace != nullptr && lane == 0u && leader)
       a.trace[step] = step_trace(P, prop_score, accept, new_max);
+    if (TRACE && a.prop_am != nullptr && leader)
+      write_rows<NS>(a.prop_am + 2ull * step * N, a.prop_dm + 2ull * step * N, pam, pdm, N);
 
     if (new_max) {
       best_score = prop_score;
@@ -564,12 +567,13 @@ struct ChainLds {
   }
 };
 
-template <int NS, int CS, int W, bool TRACE, bool INCR>
-__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
-  extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
-  const uint32_t chain = blockIdx.x;
-  if (chain >= a.n_chains) return;
+/* The view of chain `chain` of a plain launch; with TRACE, its trace
+ * records and proposal rows when it is among the traced chains. */
+template <bool TRACE>
+__device__ __forceinline__ ChainView chain_view(const ChainArgs &a, uint32_t chain) {
   const uint64_t rows = 2ull * chain * a.n_nodes;
+  const uint64_t prop_rows = rows * a.trace_stride;
+  const bool prop = TRACE && chain < a.n_prop_traced;
   ChainView v;
   v.cells = a.cells;
   v.am = a.am + rows;
@@ -583,6 +587,8 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.best_step = a.best_step + chain;
   v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
                                           : nullptr;
+  v.prop_am = prop ? a.prop_am + prop_rows : nullptr;
+  v.prop_dm = prop ? a.prop_dm + prop_rows : nullptr;
   v.seed = a.seed;
   v.job = a.job;
   v.chain = a.chain_offset + chain;
@@ -597,6 +603,15 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.gamma = a.gamma;
   v.prior = a.prior;
   v.probs = a.probs;
+  return v;
+}
+
+template <int NS, int CS, int W, bool TRACE, bool INCR>
+__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
+  extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
+  ChainView v = chain_view<TRACE>(a, chain);
   if constexpr (INCR) {
     const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
@@ -828,6 +843,8 @@ __device__ __forceinline__ void run_chain_spec(const ChainView &a, float *dyn, c
     }
     if (TRACE && a.trace != nullptr && lane == 0u && wave < committed)
       a.trace[my] = step_trace(P, prop_score, wave == j, my_max);
+    if (TRACE && a.prop_am != nullptr && wave < committed)
+      write_rows<NS>(a.prop_am + 2ull * my * N, a.prop_dm + 2ull * my * N, P.pam, P.pdm, N);
     if (wave == last_max) write_rows<NS>(a.best_am, a.best_dm, P.pam, P.pdm, N);
     if (j < n_act) {
       /* the accepting wave publishes its tree and brings the table to it:
@@ -884,34 +901,7 @@ __global__ __launch_bounds__(512) void chain_spec_kernel(ChainArgs a) {
   const uint32_t chain = blockIdx.x;
   if (chain >= a.n_chains) return;
   const uint32_t K = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  const uint64_t rows = 2ull * chain * a.n_nodes;
-  ChainView v;
-  v.cells = a.cells;
-  v.am = a.am + rows;
-  v.dm = a.dm + rows;
-  v.best_am = a.best_am + rows;
-  v.best_dm = a.best_dm + rows;
-  v.beta = a.beta + chain;
-  v.score = a.score + chain;
-  v.best_beta = a.best_beta + chain;
-  v.best_score = a.best_score + chain;
-  v.best_step = a.best_step + chain;
-  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
-                                          : nullptr;
-  v.seed = a.seed;
-  v.job = a.job;
-  v.chain = a.chain_offset + chain;
-  v.step_begin = a.step_begin;
-  v.step_end = a.step_end;
-  v.n_cells = a.n_cells;
-  v.n_nodes = a.n_nodes;
-  v.is_first_chain = a.chain_offset + chain == 0u;
-  v.score_init = a.score_init != 0u;
-  v.L00 = a.L00;
-  v.L10 = a.L10;
-  v.gamma = a.gamma;
-  v.prior = a.prior;
-  v.probs = a.probs;
+  ChainView v = chain_view<TRACE>(a, chain);
   run_chain_spec<NS, CS, W, TRACE>(v, dyn, SpecLds::of(a.n_nodes, a.n_cells, CS, K), wave, K);
 }
 
@@ -933,34 +923,9 @@ __global__ __launch_bounds__(64 * SPLIT) void chain_split_kernel(ChainArgs a) {
   /* the sums past n_cells stay +0.0 */
   for (uint32_t i = threadIdx.x; i < 256u; i += 64u * (uint32_t)SPLIT) dyn[i] = 0.0f;
   __syncthreads();
-  const uint64_t rows = 2ull * chain * a.n_nodes;
-  ChainView v;
+  ChainView v = chain_view<TRACE>(a, chain);
   v.cells = a.cells + 4ull * c0;
-  v.am = a.am + rows;
-  v.dm = a.dm + rows;
-  v.best_am = a.best_am + rows;
-  v.best_dm = a.best_dm + rows;
-  v.beta = a.beta + chain;
-  v.score = a.score + chain;
-  v.best_beta = a.best_beta + chain;
-  v.best_score = a.best_score + chain;
-  v.best_step = a.best_step + chain;
-  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
-                                          : nullptr;
-  v.seed = a.seed;
-  v.job = a.job;
-  v.chain = a.chain_offset + chain;
-  v.step_begin = a.step_begin;
-  v.step_end = a.step_end;
   v.n_cells = mine;
-  v.n_nodes = a.n_nodes;
-  v.is_first_chain = a.chain_offset + chain == 0u;
-  v.score_init = a.score_init != 0u;
-  v.L00 = a.L00;
-  v.L10 = a.L10;
-  v.gamma = a.gamma;
-  v.prior = a.prior;
-  v.probs = a.probs;
   const ChainLds l = ChainLds::of(a.n_nodes, cs, 1);
   float *slab = dyn + 256u + wave * l.total;
   WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
@@ -1006,6 +971,7 @@ batch_items_kernel(BatchArgs a, uint32_t first, uint32_t count, uint32_t wave_fl
   v.best_score = a.best_score + gc;
   v.best_step = a.best_step + gc;
   v.trace = nullptr;
+  v.prop_am = v.prop_dm = nullptr;
   v.seed = J.seed;
   v.job = J.job_tag;
   v.chain = c;

@@ -54,8 +54,9 @@ struct bbs_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf cells, cells32, am, dm, beta, score, best_am, best_dm, best_beta,
-      best_score, best_step, trace;
+      best_score, best_step, trace, prop_am, prop_dm;
   uint32_t n_traced = 0;
+  uint32_t n_prop_traced = 0; /* chains whose proposals are recorded */
   int scoring_mode = BBS_SCORING_AUTO;
   uint32_t chain_split = 0u; /* waves per chain: 0 auto, 1, 2, 4 */
   uint32_t speculation = 0u; /* speculative waves per chain: 0 auto, 1, 2, 4, 8 */
@@ -429,7 +430,8 @@ void bbs_destroy(bbs_ctx *ctx) {
   DevBuf *bufs[] = {&ctx->cells,     &ctx->cells32,    &ctx->am,         &ctx->dm,
                     &ctx->beta,      &ctx->score,      &ctx->best_am,
                     &ctx->best_dm,   &ctx->best_beta,  &ctx->best_score,
-                    &ctx->best_step, &ctx->trace};
+                    &ctx->best_step, &ctx->trace,     &ctx->prop_am,
+                    &ctx->prop_dm};
   for (DevBuf *b : bufs) b->release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -468,6 +470,21 @@ int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains) {
                          sizeof(bbs_step_trace));
   if (rc) return rc;
   ctx->n_traced = n_traced_chains;
+  return BBS_OK;
+}
+
+int bbs_set_proposal_trace(bbs_ctx *ctx, uint32_t n_traced_chains) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (n_traced_chains > ctx->params.n_chains)
+    return fail(ctx, BBS_ERR_ARG, "cannot trace %u of %u chains", n_traced_chains,
+                ctx->params.n_chains);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)n_traced_chains * ctx->params.n_steps * ctx->n_nodes * 2u *
+                       sizeof(uint64_t);
+  int rc;
+  if ((rc = dev_alloc(ctx, ctx->prop_am, bytes)) || (rc = dev_alloc(ctx, ctx->prop_dm, bytes)))
+    return rc;
+  ctx->n_prop_traced = n_traced_chains;
   return BBS_OK;
 }
 
@@ -624,6 +641,9 @@ int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   a.trace = (bbs_step_trace *)ctx->trace.p;
   a.n_traced = ctx->n_traced;
   a.trace_stride = ctx->params.n_steps;
+  a.prop_am = (uint64_t *)ctx->prop_am.p;
+  a.prop_dm = (uint64_t *)ctx->prop_dm.p;
+  a.n_prop_traced = ctx->n_prop_traced;
   a.seed = ctx->params.seed;
   a.job = 0u;
   a.chain_offset = 0u;
@@ -641,13 +661,14 @@ int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   const uint32_t table = chain_table(ctx);
   const uint32_t spec = chain_speculation(ctx);
   const uint32_t split = spec > 1u ? 1u : chain_split(ctx);
+  const bool traced = ctx->n_traced > 0u || ctx->n_prop_traced > 0u;
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (a.n_chains > 0u && spec > 1u)
-    HIP_TRY(ctx, bbs::launch_chain_spec(a, ctx->n_traced > 0u, spec, ctx->stream));
+    HIP_TRY(ctx, bbs::launch_chain_spec(a, traced, spec, ctx->stream));
   else if (a.n_chains > 0u && split > 1u)
-    HIP_TRY(ctx, bbs::launch_chain_split(a, ctx->n_traced > 0u, split, ctx->stream));
+    HIP_TRY(ctx, bbs::launch_chain_split(a, traced, split, ctx->stream));
   else if (a.n_chains > 0u)
-    HIP_TRY(ctx, bbs::launch_chain(a, ctx->n_traced > 0u, table, ctx->stream));
+    HIP_TRY(ctx, bbs::launch_chain(a, traced, table, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -731,6 +752,17 @@ int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out) {
                          (size_t)ctx->n_traced * ctx->params.n_steps *
                              sizeof(bbs_step_trace),
                          hipMemcpyDeviceToHost));
+  return BBS_OK;
+}
+
+int bbs_get_proposal_trace(bbs_ctx *ctx, uint64_t *am, uint64_t *dm) {
+  if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
+  if (!ctx->ran) return fail(ctx, BBS_ERR_STATE, "run not finished");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = (size_t)ctx->n_prop_traced * ctx->params.n_steps * ctx->n_nodes * 2u *
+                       sizeof(uint64_t);
+  if (am && bytes) HIP_TRY(ctx, hipMemcpy(am, ctx->prop_am.p, bytes, hipMemcpyDeviceToHost));
+  if (dm && bytes) HIP_TRY(ctx, hipMemcpy(dm, ctx->prop_dm.p, bytes, hipMemcpyDeviceToHost));
   return BBS_OK;
 }
 

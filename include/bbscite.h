@@ -107,6 +107,14 @@ int bbs_set_initial_states(bbs_ctx *ctx, const uint64_t *am, const uint64_t *dm,
 /* Record a bbs_step_trace for every step of chains [0, n_traced_chains). */
 int bbs_set_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
 
+/* Record the proposed tree of every step of chains [0, n_traced_chains),
+ * accepted or not: its ancestor and descendant rows as the device built
+ * them, so that each move's update can be checked matrix by matrix (the
+ * reference's move test checks the full matrices after every update,
+ * test/MutationTree.cpp:697-735; Application itself has no such record).
+ * Device memory: 32 * n_nodes bytes per traced step. */
+int bbs_set_proposal_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
+
 /* How the chain kernel scores proposals; results are bit-identical in every
  * mode (no reference counterpart: the reference rescores every pair,
  * src/TreeScorer.hpp:131-210). FULL rescores all (node, cell) pairs each
@@ -187,6 +195,11 @@ int bbs_get_chain_states(bbs_ctx *ctx, uint64_t *am, uint64_t *dm);
 
 /* Copy the recorded trace: [n_traced_chains][n_steps] records. */
 int bbs_get_trace(bbs_ctx *ctx, bbs_step_trace *out);
+
+/* Copy the recorded proposals: am/dm [n_traced_chains][n_steps][n_nodes]
+ * rows (each row two 64-bit words, as bbs_get_chain_states); either may be
+ * NULL. */
+int bbs_get_proposal_trace(bbs_ctx *ctx, uint64_t *am, uint64_t *dm);
 
 #define BBS_SCORE_NO_PRIOR 1u
 /* Use the lane-per-cell scorer (the chain kernel's) instead of the bulk

@@ -476,6 +476,15 @@ void orc_run_chain(const bbs_params *p, const orc_scorer *s, uint32_t job,
                    uint32_t chain, int is_first_chain, const orc_tree *init,
                    float init_score, orc_chain_result *res,
                    bbs_step_trace *trace) {
+  orc_run_chain_proposals(p, s, job, chain, is_first_chain, init, init_score, res,
+                          trace, NULL, NULL);
+}
+
+void orc_run_chain_proposals(const bbs_params *p, const orc_scorer *s,
+                             uint32_t job, uint32_t chain, int is_first_chain,
+                             const orc_tree *init, float init_score,
+                             orc_chain_result *res, bbs_step_trace *trace,
+                             uint64_t *prop_am, uint64_t *prop_dm) {
   static __thread orc_tree cur, prop;
   uint32_t n = s->n_genes + 1;
   if (init)
@@ -491,6 +500,14 @@ void orc_run_chain(const bbs_params *p, const orc_scorer *s, uint32_t job,
     bbs_raw_move raw = bbs_draw_move(p->seed, job, chain, step, n, probs);
     orc_mod mod = orc_realize(&cur, &raw);
     orc_update(&cur, &mod, &prop);
+    for (uint32_t x = 0; prop_am && x < n; x++) {
+      uint64_t *a = prop_am + 2u * ((size_t)step * n + x);
+      uint64_t *d = prop_dm + 2u * ((size_t)step * n + x);
+      a[0] = prop.am[x].w[0];
+      a[1] = prop.am[x].w[1];
+      d[0] = prop.dm[x].w[0];
+      d[1] = prop.dm[x].w[1];
+    }
     float prop_score = orc_logscore_tree(s, &prop, 1, NULL, NULL);
     float corr;
     if (raw.move_type == BBS_MOVE_SWAP_SUBTREES &&

@@ -124,6 +124,15 @@ void orc_run_chain(const bbs_params *p, const orc_scorer *s, uint32_t job,
                    float init_score, orc_chain_result *res,
                    bbs_step_trace *trace);
 
+/* orc_run_chain that also records every step's proposed tree (accepted or
+ * not): prop_am / prop_dm [n_steps][n_nodes][2] words, both or neither
+ * NULL. */
+void orc_run_chain_proposals(const bbs_params *p, const orc_scorer *s,
+                             uint32_t job, uint32_t chain, int is_first_chain,
+                             const orc_tree *init, float init_score,
+                             orc_chain_result *res, bbs_step_trace *trace,
+                             uint64_t *prop_am, uint64_t *prop_dm);
+
 /* Whole run, chain-major (P = 1): the global best as the reference keeps it. */
 void orc_run(const bbs_params *p, const orc_scorer *s, uint32_t job,
              orc_tree *best_tree, float *best_score, uint32_t *best_chain,

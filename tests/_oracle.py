@@ -132,6 +132,22 @@ _SIGS = {
             _P,
         ],
     ),
+    "orc_run_chain_proposals": (
+        None,
+        [
+            C.POINTER(Params),
+            C.POINTER(Scorer),
+            _u32,
+            _u32,
+            C.c_int,
+            C.POINTER(Tree),
+            C.c_float,
+            C.POINTER(ChainResult),
+            _P,
+            _P,
+            _P,
+        ],
+    ),
     "orc_run": (
         None,
         [
@@ -267,6 +283,20 @@ def run_chain(p: Params, s: Scorer, chain, is_first, job=0, init=None, init_scor
         C.byref(init) if init is not None else None, init_score, C.byref(res), _ptr(tr),
     )
     return (res, tr) if trace else res
+
+
+def run_chain_proposals(p: Params, s: Scorer, chain, is_first, n_nodes, job=0):
+    """orc_run_chain with every step's proposed tree: (result, trace, am,
+    dm), am/dm [n_steps][n_nodes][2] uint64 rows."""
+    res = ChainResult()
+    tr = np.zeros(p.n_steps, TRACE_DTYPE)
+    am = np.zeros((p.n_steps, n_nodes, 2), np.uint64)
+    dm = np.zeros((p.n_steps, n_nodes, 2), np.uint64)
+    lib().orc_run_chain_proposals(
+        C.byref(p), C.byref(s), job, chain, int(is_first), None, math.nan, C.byref(res),
+        _ptr(tr), _ptr(am), _ptr(dm),
+    )
+    return res, tr, am, dm
 
 
 def run(p: Params, s: Scorer, job=0):

@@ -196,8 +196,10 @@ def test_chain_trace_replay(case, mode):
     with B.Context(p, mut, kno, G) as ctx:
         _set_mode(ctx, mode)
         ctx.set_trace(R)
+        ctx.set_proposal_trace(R)
         ctx.run()
         trace = ctx.trace(R)
+        prop_am, prop_dm = ctx.proposal_trace(R)
         bs, st, fs, fb = ctx.chain_results()
         am_f, dm_f = ctx.chain_states()
         best_am, best_dm, best_beta, best_score = ctx.best()
@@ -214,7 +216,13 @@ def test_chain_trace_replay(case, mode):
     assert np.array_equal(plain[2][1], best_dm) and _bits(plain[2][3]) == _bits(best_score)
     s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
     for c in range(R):
-        res, otr = O.run_chain(p, s, c, c == 0, trace=True)
+        res, otr, oam, odm = O.run_chain_proposals(p, s, c, c == 0, G + 1)
+        # every step's proposed tree, accepted or not, matrix for matrix
+        # (the reference's move test checks the full matrices after each
+        # update, test/MutationTree.cpp:697-735)
+        for name, got, want in (("am", prop_am[c], oam), ("dm", prop_dm[c], odm)):
+            bad = np.nonzero((got != want).any(axis=(1, 2)))[0]
+            assert len(bad) == 0, (c, name, bad[:5], trace[c][bad[:3]])
         for field in ("move_type", "v", "w", "parent_of_v", "parent_of_w",
                       "descendant_of_v", "nondescendant_of_v", "flags"):
             bad = np.nonzero(trace[c][field] != otr[field])[0]
@@ -370,6 +378,12 @@ def test_errors_are_reported():
         assert e.value.code == B.ERR_STATE
         with pytest.raises(B.BbsError):
             ctx.set_trace(5)
+        with pytest.raises(B.BbsError) as e:
+            ctx.set_proposal_trace(3)
+        assert e.value.code == B.ERR_ARG
+        with pytest.raises(B.BbsError) as e:
+            ctx.proposal_trace(0)
+        assert e.value.code == B.ERR_STATE
         ctx.begin()
         with pytest.raises(B.BbsError):
             ctx.advance(11)

@@ -173,3 +173,43 @@ def test_node_pair_draw_is_uniform_over_nonroot_pairs():
         row = counts[v][off[v]]
         ev = row.sum() / (n - 2)
         assert float(((row - ev) ** 2 / ev).sum()) < 40.0  # 9 dof, p ~ 4e-6
+
+
+def test_recorded_proposals_are_each_steps_move():
+    """The oracle's per-step record of proposed trees, which the device's
+    record is compared with row for row: each step's rows are the tree that
+    the step's move makes of the current tree (the parent-vector
+    restatement of test/MutationTree.cpp:697-735), a beta move proposes the
+    current tree, and the current tree follows the accepted steps."""
+    m, _, _ = B.generate_dataset(21, 24, 30, 0.05, 0.2, 0.1)
+    mut, kno = B.encode_matrix(m)
+    G = m.shape[0]
+    n = G + 1
+    p = B.default_params(n_chains=1, n_steps=600, seed=21, alpha_mean=0.05, beta_mean=0.2,
+                         prob_beta_change=0.2)
+    s = O.make_scorer(p.alpha_mean, p.beta_mean, p.beta_sd, mut, kno, G)
+    res, tr, am, dm = O.run_chain_proposals(p, s, 0, True, n)
+    t0 = O.Tree()
+    O.lib().orc_initial_tree(C.byref(p), 0, 0, G, C.byref(t0))
+    parent = O.parents(t0)
+    accepted = [0, 0, 0, 0]
+    for step in range(p.n_steps):
+        rec = tr[step]
+        if rec["move_type"] == 0:
+            expected = parent
+        else:
+            mod = O.Mod()
+            for k in ("move_type", "v", "w", "parent_of_v", "parent_of_w",
+                      "descendant_of_v", "nondescendant_of_v"):
+                setattr(mod, k, int(rec[k]))
+            expected = _manual_move(parent, mod, O.tree_from_parent(parent))
+        ref = O.tree_from_parent(expected)
+        assert np.array_equal(am[step], O.rows(ref, "am")), step
+        assert np.array_equal(dm[step], O.rows(ref, "dm")), step
+        if rec["flags"] & 1:
+            parent = expected
+            accepted[int(rec["move_type"])] += 1
+    final = O.tree_from_parent(parent)
+    assert np.array_equal(O.rows(res.final_tree, "dm"), O.rows(final, "dm"))
+    # non-vacuous: accepted moves of every kind change the current tree
+    assert min(accepted) >= 5, accepted
