@@ -28,8 +28,6 @@ struct ChainArgs {
   uint32_t *best_step;    /* UINT32_MAX: no record yet                */
   bbs_step_trace *trace;  /* [n_traced][trace_stride]                 */
   uint32_t n_traced, trace_stride;
-  uint64_t *prop_am, *prop_dm; /* [n_prop_traced][trace_stride][n_nodes][2] */
-  uint32_t n_prop_traced;
   uint64_t seed;
   uint32_t job, chain_offset;
   uint32_t n_chains, step_begin, step_end;
@@ -38,6 +36,8 @@ struct ChainArgs {
   float L00, L10, gamma;
   PriorArgs prior;
   bbs_move_probs probs;
+  uint64_t *prop_am, *prop_dm; /* [n_prop_traced][trace_stride][n_nodes][2] */
+  uint32_t n_prop_traced;
 };
 
 struct ScoreArgs {

@@ -567,13 +567,12 @@ struct ChainLds {
   }
 };
 
-/* The view of chain `chain` of a plain launch; with TRACE, its trace
- * records and proposal rows when it is among the traced chains. */
-template <bool TRACE>
-__device__ __forceinline__ ChainView chain_view(const ChainArgs &a, uint32_t chain) {
+template <int NS, int CS, int W, bool TRACE, bool INCR>
+__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
+  extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
+  const uint32_t chain = blockIdx.x;
+  if (chain >= a.n_chains) return;
   const uint64_t rows = 2ull * chain * a.n_nodes;
-  const uint64_t prop_rows = rows * a.trace_stride;
-  const bool prop = TRACE && chain < a.n_prop_traced;
   ChainView v;
   v.cells = a.cells;
   v.am = a.am + rows;
@@ -587,8 +586,8 @@ __device__ __forceinline__ ChainView chain_view(const ChainArgs &a, uint32_t cha
   v.best_step = a.best_step + chain;
   v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
                                           : nullptr;
-  v.prop_am = prop ? a.prop_am + prop_rows : nullptr;
-  v.prop_dm = prop ? a.prop_dm + prop_rows : nullptr;
+  v.prop_am = (TRACE && chain < a.n_prop_traced) ? a.prop_am + rows * a.trace_stride : nullptr;
+  v.prop_dm = (TRACE && chain < a.n_prop_traced) ? a.prop_dm + rows * a.trace_stride : nullptr;
   v.seed = a.seed;
   v.job = a.job;
   v.chain = a.chain_offset + chain;
@@ -603,15 +602,6 @@ __device__ __forceinline__ ChainView chain_view(const ChainArgs &a, uint32_t cha
   v.gamma = a.gamma;
   v.prior = a.prior;
   v.probs = a.probs;
-  return v;
-}
-
-template <int NS, int CS, int W, bool TRACE, bool INCR>
-__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
-  extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
-  const uint32_t chain = blockIdx.x;
-  if (chain >= a.n_chains) return;
-  ChainView v = chain_view<TRACE>(a, chain);
   if constexpr (INCR) {
     const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
@@ -901,7 +891,36 @@ __global__ __launch_bounds__(512) void chain_spec_kernel(ChainArgs a) {
   const uint32_t chain = blockIdx.x;
   if (chain >= a.n_chains) return;
   const uint32_t K = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  ChainView v = chain_view<TRACE>(a, chain);
+  const uint64_t rows = 2ull * chain * a.n_nodes;
+  ChainView v;
+  v.cells = a.cells;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + chain;
+  v.score = a.score + chain;
+  v.best_beta = a.best_beta + chain;
+  v.best_score = a.best_score + chain;
+  v.best_step = a.best_step + chain;
+  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
+                                          : nullptr;
+  v.prop_am = (TRACE && chain < a.n_prop_traced) ? a.prop_am + rows * a.trace_stride : nullptr;
+  v.prop_dm = (TRACE && chain < a.n_prop_traced) ? a.prop_dm + rows * a.trace_stride : nullptr;
+  v.seed = a.seed;
+  v.job = a.job;
+  v.chain = a.chain_offset + chain;
+  v.step_begin = a.step_begin;
+  v.step_end = a.step_end;
+  v.n_cells = a.n_cells;
+  v.n_nodes = a.n_nodes;
+  v.is_first_chain = a.chain_offset + chain == 0u;
+  v.score_init = a.score_init != 0u;
+  v.L00 = a.L00;
+  v.L10 = a.L10;
+  v.gamma = a.gamma;
+  v.prior = a.prior;
+  v.probs = a.probs;
   run_chain_spec<NS, CS, W, TRACE>(v, dyn, SpecLds::of(a.n_nodes, a.n_cells, CS, K), wave, K);
 }
 
@@ -923,9 +942,36 @@ __global__ __launch_bounds__(64 * SPLIT) void chain_split_kernel(ChainArgs a) {
   /* the sums past n_cells stay +0.0 */
   for (uint32_t i = threadIdx.x; i < 256u; i += 64u * (uint32_t)SPLIT) dyn[i] = 0.0f;
   __syncthreads();
-  ChainView v = chain_view<TRACE>(a, chain);
+  const uint64_t rows = 2ull * chain * a.n_nodes;
+  ChainView v;
   v.cells = a.cells + 4ull * c0;
+  v.am = a.am + rows;
+  v.dm = a.dm + rows;
+  v.best_am = a.best_am + rows;
+  v.best_dm = a.best_dm + rows;
+  v.beta = a.beta + chain;
+  v.score = a.score + chain;
+  v.best_beta = a.best_beta + chain;
+  v.best_score = a.best_score + chain;
+  v.best_step = a.best_step + chain;
+  v.trace = (TRACE && chain < a.n_traced) ? a.trace + (uint64_t)chain * a.trace_stride
+                                          : nullptr;
+  v.prop_am = (TRACE && chain < a.n_prop_traced) ? a.prop_am + rows * a.trace_stride : nullptr;
+  v.prop_dm = (TRACE && chain < a.n_prop_traced) ? a.prop_dm + rows * a.trace_stride : nullptr;
+  v.seed = a.seed;
+  v.job = a.job;
+  v.chain = a.chain_offset + chain;
+  v.step_begin = a.step_begin;
+  v.step_end = a.step_end;
   v.n_cells = mine;
+  v.n_nodes = a.n_nodes;
+  v.is_first_chain = a.chain_offset + chain == 0u;
+  v.score_init = a.score_init != 0u;
+  v.L00 = a.L00;
+  v.L10 = a.L10;
+  v.gamma = a.gamma;
+  v.prior = a.prior;
+  v.probs = a.probs;
   const ChainLds l = ChainLds::of(a.n_nodes, cs, 1);
   float *slab = dyn + 256u + wave * l.total;
   WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};

@@ -507,6 +507,35 @@ def few_chains_bench():
     return out
 
 
+def full_device_split_bench(steps=20_000):
+    """Waves per chain with every SIMD busy: one full round of chains (four
+    per CU, or three where the split form's LDS allows no more) with one
+    wave per chain, against each chain's cells split over two and four
+    waves (two and four waves per SIMD)."""
+    out = []
+    for G, C, chains in ((80, 82, 1024), (94, 95, 768), (64, 96, 1024), (90, 70, 1024)):
+        m, _, _ = B.generate_dataset(7, G, C, 1e-4, 0.3, 0.15)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=chains, n_steps=steps, seed=7, alpha_mean=1e-4,
+                             beta_mean=0.3, beta_sd=0.05, prob_beta_change=0.1)
+        row = {"genes": G, "cells": C, "chains": chains, "steps": steps}
+        best = set()
+        for split in (1, 2, 4):
+            with B.Context(p, mut, kno, G) as ctx:
+                ctx.set_speculation(1)
+                ctx.set_chain_split(split)
+                ctx.run()  # first launch of the instance
+                ms = ctx.run()
+                best.add(np.float32(ctx.best()[3]).tobytes())
+            row[f"split_{split}_ms"] = round(ms, 2)
+            row[f"split_{split}_steps_per_s"] = chains * steps / (ms / 1e3)
+        row["best_ll_identical"] = len(best) == 1
+        out.append(row)
+        log(f"[bench] {G}x{C} {chains} chains: " + ", ".join(
+            f"split {k} {row[f'split_{k}_ms']:.1f} ms" for k in (1, 2, 4)))
+    return out
+
+
 def layout_code(genes, cells):
     """Register layout (node slots, cell slots, gene words) as the library's
     shape_of picks it, as the code NS*100 + CS*10 + W."""
@@ -627,6 +656,8 @@ def main():
     ap.add_argument("--serve-cap", type=int, default=20_000)
     ap.add_argument("--round-ms", type=float, default=None,
                     help="server slice length (default: the library's)")
+    ap.add_argument("--split-full-device", action="store_true",
+                    help="only time waves per chain with every SIMD busy (one JSON line)")
     ap.add_argument("--valu-peak-only", action="store_true",
                     help="only run the int32 VALU probe (for its counters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -642,6 +673,9 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         log(f"[bench] copied {a.copy_calibration} MiB")
+        return
+    if a.split_full_device:
+        print(json.dumps({"split_full_device": full_device_split_bench()}), flush=True)
         return
     if a.valu_peak_only:
         m0, _, _ = B.generate_dataset(1, 4, 4, 1e-4, 0.2, 0.1)
