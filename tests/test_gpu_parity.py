@@ -280,18 +280,28 @@ def test_advance_windows_equal_one_run():
     say("[windows]")
     p, mut, kno = _chain_case(30, 40, 9, 8, 300, 0.1)
     with B.Context(p, mut, kno, 30) as ctx:
+        ctx.set_trace(3)
+        ctx.set_proposal_trace(3)
         ctx.run()
         one = ctx.chain_results(), ctx.chain_states(), ctx.best()
+        one_tr = ctx.trace(3), ctx.proposal_trace(3)
     with B.Context(p, mut, kno, 30) as ctx:
+        ctx.set_trace(3)
+        ctx.set_proposal_trace(3)
         ctx.begin()
         for n in (1, 99, 100, 100):
             ctx.advance(n)
         split = ctx.chain_results(), ctx.chain_states(), ctx.best()
+        split_tr = ctx.trace(3), ctx.proposal_trace(3)
     for a, b in zip(one[0], split[0]):
         assert np.array_equal(a, b)
     for a, b in zip(one[1], split[1]):
         assert np.array_equal(a, b)
     assert np.array_equal(one[2][0], split[2][0]) and one[2][3] == split[2][3]
+    # the records of every step land at the same place across windows
+    assert np.array_equal(one_tr[0], split_tr[0])
+    for a, b in zip(one_tr[1], split_tr[1]):
+        assert np.array_equal(a, b)
 
 
 def test_caller_initial_states():
