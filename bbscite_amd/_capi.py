@@ -125,7 +125,13 @@ class ServerConfig(C.Structure):
         ("waves_per_group", C.c_uint32),
         ("n_streams", C.c_uint32),
         ("two_wave_layouts", C.c_uint32),
+        ("admission", C.c_uint32),
+        ("aging_s", C.c_float),
     ]
+
+
+ADMIT_ARRIVAL = 0
+ADMIT_LEAST_WORK = 1
 
 
 class JobResult(C.Structure):

@@ -2053,8 +2053,6 @@ void srv_free_job(bbs_server *s, SrvJob *j) {
   delete j;
 }
 
-/* Admit the head of the queue if memory and concurrent-wave capacity allow
- * (an empty server admits any job, so none waits forever). */
 /* The lane of a layout, created on first use: streams are dealt out in
  * creation order, and the job and item tables are sized for the most a
  * lane can hold (chain_cap chains), so they never grow under a slice. */
@@ -2078,9 +2076,35 @@ int srv_lane(bbs_server *s, uint32_t shape, Lane *&out) {
   return BBS_OK;
 }
 
+/* The pending job that admission considers next: the oldest, or the one of
+ * least predicted work (chains x steps x step cost) aged by its time
+ * pending: work x exp(-(now - submitted) / aging_s), the oldest of equals.
+ * The aged order does not change while jobs wait (the factor exp(-now /
+ * aging_s) is common to all), so it is the order of ln(work) + submitted /
+ * aging_s. */
+size_t srv_next(const bbs_server *s) {
+  if (s->cfg.admission != BBS_ADMIT_LEAST_WORK) return 0u;
+  const double inv_aging = s->cfg.aging_s > 0.0f ? 1.0 / s->cfg.aging_s : 0.0;
+  size_t best = 0u;
+  double best_key = 0.0;
+  for (size_t i = 0; i < s->pending.size(); i++) {
+    const SrvJob *j = s->pending[i];
+    const double work = (double)j->J.n_chains * j->n_steps * j->cost;
+    const double key = std::log(work) + j->t_submit * inv_aging;
+    if (i == 0u || key < best_key) {
+      best = i;
+      best_key = key;
+    }
+  }
+  return best;
+}
+
+/* Admit the next pending job (srv_next) if memory and concurrent-wave
+ * capacity allow (an empty server admits any job, so none waits forever). */
 int srv_admit(bbs_server *s, bool &admitted) {
   admitted = false;
-  SrvJob *j = s->pending.front();
+  const size_t at = srv_next(s);
+  SrvJob *j = s->pending[at];
   bbs::BatchJob &J = j->J;
   const bool empty = s->n_resident == 0u;
   if (!empty && (s->chains_used + J.n_chains > s->chain_cap ||
@@ -2110,7 +2134,7 @@ int srv_admit(bbs_server *s, bool &admitted) {
   SHIP_TRY(s, hipMemcpyAsync((uint64_t *)s->cells.p + 4ull * x, j->cells.data(),
                              j->cells.size() * 8u, hipMemcpyHostToDevice, l->stream));
   j->t_admit = srv_now(s);
-  s->pending.pop_front();
+  s->pending.erase(s->pending.begin() + (std::ptrdiff_t)at);
   l->jobs.push_back(j);
   s->n_resident++;
   admitted = true;
@@ -2330,6 +2354,8 @@ void bbs_server_config_default(bbs_server_config *cfg) {
   cfg->waves_per_group = 1u;
   cfg->n_streams = 4u;
   cfg->two_wave_layouts = BBS_TWO_WAVE_DEFAULT;
+  cfg->admission = BBS_ADMIT_ARRIVAL;
+  cfg->aging_s = 0.0f;
 }
 
 const char *bbs_server_last_error(const bbs_server *s) {
@@ -2369,7 +2395,9 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out) {
     bbs_server_config_default(&c);
   if (c.waves_per_group < 1u || c.waves_per_group > 4u || c.n_streams < 1u ||
       c.n_streams > (uint32_t)BBS_MAX_STREAMS || c.round_cost == 0u || !(c.round_ms > 0.0f) ||
-      c.hbm_budget_bytes < (1ull << 20))
+      c.hbm_budget_bytes < (1ull << 20) ||
+      (c.admission != BBS_ADMIT_ARRIVAL && c.admission != BBS_ADMIT_LEAST_WORK) ||
+      !(c.aging_s >= 0.0f))
     return sfail(nullptr, BBS_ERR_ARG, "invalid server configuration");
   int n_dev = 0;
   if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0)
@@ -2496,7 +2524,7 @@ int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,
       if (rc) return rc;
       progressed = true;
     }
-    /* admission in arrival order */
+    /* admission in the configured order */
     while (!s->pending.empty()) {
       bool ok = false;
       const int rc = srv_admit(s, ok);

@@ -320,7 +320,10 @@ def config5_jobs(n_jobs, seed=5):
     return WL.served_stream(n_jobs, seed)
 
 
-def serving_bench(n_jobs, latencies_path=None, deadline_s=None):
+ADMISSION_NAMES = {B.ADMIT_ARRIVAL: "arrival order", B.ADMIT_LEAST_WORK: "least work first"}
+
+
+def serving_bench(n_jobs, latencies_path=None, deadline_s=None, admission=None, aging_s=0.0):
     """Continuous batching on the first n_jobs of the served stream, in real
     time: each job is submitted at its arrival time; latency = completion -
     arrival. With deadline_s the run stops after that many seconds and
@@ -330,8 +333,9 @@ def serving_bench(n_jobs, latencies_path=None, deadline_s=None):
     lat = []
     arrival = {}
     records = []
-    with B.Server(B.server_config(hbm_budget_bytes=4 << 30, n_streams=16,
-                                  round_ms=SERVE_ROUND_MS)) as srv:
+    kw = {} if admission is None else {"admission": admission, "aging_s": aging_s}
+    cfg = B.server_config(hbm_budget_bytes=4 << 30, n_streams=16, round_ms=SERVE_ROUND_MS, **kw)
+    with B.Server(cfg) as srv:
         t0 = srv.now()
         nxt = 0
         done = 0
@@ -376,6 +380,8 @@ def serving_bench(n_jobs, latencies_path=None, deadline_s=None):
         "jobs_completed": done,
         "complete": complete,
         "subset_of_10000_job_stream": n_jobs < 10000,
+        "admission": ADMISSION_NAMES[cfg.admission],
+        "aging_s": cfg.aging_s,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p90_ms": float(np.percentile(lat, 90) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -648,6 +654,10 @@ def main():
     ap.add_argument("--batch-shapes", default=None, metavar="WPG:STREAMS,...",
                     help="only time the 256-dataset batch at --batch-steps steps with each "
                          "given launch shape (waves per group : streams), no autotuning")
+    ap.add_argument("--serve-admission", choices=("arrival", "least-work"), default=None,
+                    help="server admission order (default: the library's)")
+    ap.add_argument("--serve-aging", type=float, default=0.0, metavar="S",
+                    help="with --serve-admission least-work: aging time constant (s)")
     ap.add_argument("--serve-deadline", type=float, default=None, metavar="S",
                     help="with --serve-only: stop after S seconds and report what completed")
     ap.add_argument("--serve-vs-batch", type=int, default=0, metavar="N",
@@ -702,8 +712,10 @@ def main():
         print(json.dumps(serve_vs_batch(a.serve_vs_batch, a.serve_cap, a.round_ms)), flush=True)
         return
     if a.serve_only:
-        print(json.dumps(serving_bench(a.serve_jobs, a.serve_latencies, a.serve_deadline)),
-              flush=True)
+        adm = {None: None, "arrival": B.ADMIT_ARRIVAL,
+               "least-work": B.ADMIT_LEAST_WORK}[a.serve_admission]
+        print(json.dumps(serving_bench(a.serve_jobs, a.serve_latencies, a.serve_deadline,
+                                       adm, a.serve_aging)), flush=True)
         return
     if a.layout_profile:
         print(json.dumps(layout_profile(a.layout_steps, a.layout, wpg=a.layout_wpg,

@@ -382,7 +382,25 @@ typedef struct bbs_server_config {
                                    only on as many hardware queues as HIP
                                    gives the process (GPU_MAX_HW_QUEUES)   */
   uint32_t two_wave_layouts;    /* as bbs_batch_config.two_wave_layouts     */
+  uint32_t admission;           /* BBS_ADMIT_*: which pending job is
+                                   admitted next                           */
+  float aging_s;                /* BBS_ADMIT_LEAST_WORK: a job's predicted
+                                   work counts as multiplied by
+                                   exp(-(seconds pending) / aging_s), so
+                                   aging_s seconds of waiting weigh as a
+                                   factor e of work; small values approach
+                                   arrival order; 0: no aging (a steady
+                                   stream of smaller jobs can then hold a
+                                   large one back)                         */
 } bbs_server_config;
+
+/* Admission order of pending jobs (no reference counterpart: the reference
+ * runs one job per process). The policy picks the next candidate; it is
+ * admitted if it fits the budgets, else admission waits for resources. */
+#define BBS_ADMIT_ARRIVAL 0u    /* arrival order                          */
+#define BBS_ADMIT_LEAST_WORK 1u /* least predicted work first: chains x
+                                   steps x the layout's step cost (ties in
+                                   arrival order)                         */
 
 typedef struct bbs_job_result {
   uint64_t id;
@@ -399,7 +417,7 @@ int bbs_server_create(const bbs_server_config *cfg, bbs_server **out);
 /* Copies the job (parameters and data); *id identifies its result. */
 int bbs_server_submit(bbs_server *s, const bbs_job *job, uint64_t *id);
 /* Collects the lanes whose slice has ended (retiring finished jobs), admits
- * pending jobs in arrival order while they fit, and launches the next slice
+ * pending jobs in the configured order while they fit, and launches the next slice
  * of every idle lane. If none of that was possible while slices run, waits
  * for the oldest one to end first. Counts may be NULL. */
 int bbs_server_step(bbs_server *s, uint32_t *n_resident, uint32_t *n_pending,

@@ -21,7 +21,7 @@ def say(msg):
 def test_server_rejects_bad_configuration():
     """Checked before any device is touched."""
     for kw in ({"waves_per_group": 5}, {"n_streams": 0}, {"round_cost": 0},
-               {"hbm_budget_bytes": 1024}):
+               {"hbm_budget_bytes": 1024}, {"admission": 2}, {"aging_s": -1.0}):
         with pytest.raises(B.BbsError) as e:
             B.Server(B.server_config(**kw))
         assert e.value.code == _capi.ERR_ARG
@@ -43,14 +43,17 @@ JOBS = [  # G, C, chains, steps: every register layout family, uneven lengths
 
 
 @pytest.mark.gpu
-def test_served_results_equal_alone():
+@pytest.mark.parametrize("admission", [_capi.ADMIT_ARRIVAL, _capi.ADMIT_LEAST_WORK])
+def test_served_results_equal_alone(admission):
     """Small slices (many windows per job) and a small chain budget (jobs
     queue and are admitted as others finish); some jobs arrive while others
-    run. Each result equals the job's plain run bit for bit."""
-    say("[serve] mixed jobs")
+    run. Each result equals the job's plain run bit for bit, in either
+    admission order."""
+    say(f"[serve] mixed jobs, admission {admission}")
     jobs = [_job(i, *spec) for i, spec in enumerate(JOBS)]
     cfg = B.server_config(round_cost=300_000, max_resident_chains=40, n_streams=3,
-                          waves_per_group=2, hbm_budget_bytes=64 << 20)
+                          waves_per_group=2, hbm_budget_bytes=64 << 20,
+                          admission=admission)
     results = {}
     ids = {}
     with B.Server(cfg) as srv:
@@ -82,6 +85,33 @@ def test_served_results_equal_alone():
         assert _bits(r.best_beta) == _bits(beta), i
         assert np.array_equal(r.rows("am"), am) and np.array_equal(r.rows("dm"), dm), i
         assert r.best_step == st[r.best_chain]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("admission", [_capi.ADMIT_ARRIVAL, _capi.ADMIT_LEAST_WORK])
+def test_admission_order(admission):
+    """Room for one job at a time and four jobs of one layout pending,
+    submitted longest first: arrival order admits them as submitted, least
+    work first admits them shortest first."""
+    say(f"[serve] admission order {admission}")
+    steps = [800, 600, 400, 200]
+    jobs = [_job(i, 30, 40, 16, n) for i, n in enumerate(steps)]
+    cfg = B.server_config(max_resident_chains=16, n_streams=1, admission=admission)
+    done = []
+    with B.Server(cfg) as srv:
+        ids = {srv.submit(*j): i for i, j in enumerate(jobs)}
+        for _ in range(100000):
+            srv.step()
+            done += srv.poll()
+            if len(done) == len(jobs):
+                break
+    assert len(done) == len(jobs)
+    order = [ids[r.id] for r in sorted(done, key=lambda r: r.admit_s)]
+    assert order == ([0, 1, 2, 3] if admission == _capi.ADMIT_ARRIVAL else [3, 2, 1, 0])
+    # one resident at a time: each admission follows the previous completion
+    by_admit = sorted(done, key=lambda r: r.admit_s)
+    for a, b in zip(by_admit, by_admit[1:]):
+        assert a.done_s <= b.admit_s
 
 
 @pytest.mark.gpu
