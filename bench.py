@@ -778,10 +778,16 @@ def main():
     ll_gpu = score.pop("_ll_gpu") if score else None
     batched = None if a.no_batch else batched_bench(a.batch_steps)
     serving = serving_bench(a.serve_jobs) if a.serve_jobs > 0 else None
+    serving_lw = (serving_bench(a.serve_jobs, admission=B.ADMIT_LEAST_WORK)
+                  if a.serve_jobs > 0 else None)
     if serving:
         log(f"[bench] served {serving['jobs']} jobs: p50 {serving['p50_ms']:.0f} ms, "
             f"p99 {serving['p99_ms']:.0f} ms, {serving['aggregate_steps_per_s']:.3e} steps/s "
             f"(offered {serving['offered_steps_per_s']:.3e})")
+    if serving_lw:
+        log(f"[bench] served, least work first: p50 {serving_lw['p50_ms']:.0f} ms, "
+            f"p99 {serving_lw['p99_ms']:.0f} ms, "
+            f"{serving_lw['aggregate_steps_per_s']:.3e} steps/s")
     if batched:
         log(f"[bench] batched {batched['batched_steps_per_s']:.3e} steps/s, plain "
             f"{batched['plain_steps_per_s']:.3e} steps/s, x{batched['speedup']:.2f}")
@@ -834,6 +840,7 @@ def main():
         "batched": batched,
         "serving_p99_ms": serving["p99_ms"] if serving else None,
         "serving": serving,
+        "serving_least_work_first": serving_lw,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
