@@ -1036,7 +1036,11 @@ batch_items_kernel(BatchArgs a, uint32_t first, uint32_t count, uint32_t wave_fl
     float *slab = dyn + wave * wave_floats;
     const ChainLds l = ChainLds::of(J.n_nodes, J.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(slab + l.rows), slab + l.sums, slab + l.pen};
-    run_chain<NS, CS, W, false, true, false>(
+    /* layout 233 (65..96 genes, 65..96 cells): its LDS slabs hold it to
+     * about one wave per SIMD, so the matrix operands' registers cost it
+     * little concurrency, and its beta moves count pairs on the matrix
+     * cores where the dataset fills its tiles (measured faster) */
+    run_chain<NS, CS, W, false, true, NS == 2 && CS == 3 && W == 3 && WPS == 1>(
         v, lds, TableView{slab + l.table, TableView::stride_for(J.n_nodes), J.n_cells});
   } else {
     __shared__ WaveLdsStore lds_all[4];
