@@ -454,8 +454,9 @@ class Context:
         self._c(lib().bbs_set_proposal_trace(self.h, n))
 
     def set_scoring_mode(self, mode):
-        """"auto", "full" or "incremental" (results are identical)."""
-        code = {"auto": 0, "full": 1, "incremental": 2}[mode]
+        """"auto", "full", "incremental" or "half" (the table of cells 0..63,
+        over 96 cells); results are identical."""
+        code = {"auto": 0, "full": 1, "incremental": 2, "half": 3}[mode]
         self._c(lib().bbs_set_scoring_mode(self.h, code))
 
     def set_chain_split(self, waves_per_chain):

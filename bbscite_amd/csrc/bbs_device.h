@@ -507,28 +507,30 @@ __device__ __forceinline__ float finish_sum(const float (&best)[2], const bool (
  * latency, so rows are read one node pair ahead; past the last node the
  * clamped index repeats node n - 1, which rewrites its own score and cannot
  * change a maximum (nor, being equal, the first maximal node). */
-template <int NS, int CS, int W, bool COUNTS, bool TAB>
+template <int NS, int CS, int W, bool COUNTS, bool TAB, int J0 = 0, bool T0 = false>
 __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_nodes,
                                             const LogProbs &L, WaveLds &S, TableView tab,
                                             float (&best)[2], bool (&nan0)[2],
                                             uint32_t (&arg01)[2], uint32_t (&arg11)[2]) {
   static_assert(!COUNTS || CS != 3, "counts need full cell slots");
+  static_assert((J0 == 0 && !T0) || CS == 2, "slot ranges and half tables: two full slots");
   constexpr int SL = Slots<CS>::n;
   const uint32_t lane = lane_id();
   float *trow[2] = {nullptr, nullptr};
   if (TAB) {
     trow[0] = tab.row(slot_cell<CS>(0));
-    trow[1] = tab.row(slot_cell<CS>(1));
+    if (!T0) trow[1] = tab.row(slot_cell<CS>(1));
   }
-  nan0[0] = nan0[1] = false;
-  /* one node for the full slots */
+#pragma unroll
+  for (int j = J0; j < 2; j++) nan0[j] = false;
+  /* one node for the full slots (from slot J0 on) */
   auto full = [&](const uint4 &A, uint32_t node, bool first) {
 #pragma unroll
-    for (int j = 0; j < (CS == 3 ? 1 : SL); j++) {
+    for (int j = J0; j < (CS == 3 ? 1 : SL); j++) {
       uint32_t n01, n11;
       pair_counts<W>(c.km[j], c.knm[j], A, n01, n11);
       const float s = pair_score(n01, n11, c.fa[j], c.fb[j], L);
-      if (TAB) trow[j][node] = s;
+      if (TAB && !(T0 && j == 1)) trow[j][node] = s;
       if (first) {
         best[j] = s;
         nan0[j] = s != s;
@@ -739,7 +741,8 @@ __device__ __forceinline__ float mfma_maxima(const MfmaCells<CS, W> &mc, uint32_
  * LDS broadcasts. Returns the same value in every lane. With COUNTS (CS 1
  * or 2 only), n01/n11 of the first maximal node are returned per slot.
  */
-template <int NS, int CS, int W, bool COUNTS, bool TAB = false, bool SPL = false>
+template <int NS, int CS, int W, bool COUNTS, bool TAB = false, bool SPL = false,
+          bool T0 = false>
 __device__ __forceinline__ float
 score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes,
            uint32_t n_cells, const LogProbs &L, WaveLds &S,
@@ -748,7 +751,7 @@ score_tree(const uint64_t (&dm)[NS][NS], const Cells<CS, W> &c, uint32_t n_nodes
   store_rows<NS>(dm, n_nodes, S);
   float best[2];
   bool nan0[2];
-  node_maxima<NS, CS, W, COUNTS, TAB>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
+  node_maxima<NS, CS, W, COUNTS, TAB, 0, T0>(c, n_nodes, L, S, tab, best, nan0, arg01, arg11);
   if constexpr (SPL) {
     static_assert(CS == 1, "a split chain holds one cell slot per wave");
     return finish_sum_split(best[0], nan0[0], n_cells, sp);
@@ -777,17 +780,18 @@ __device__ __forceinline__ uint32_t pop_node(uint64_t (&m)[NS]) {
  * score of node 0 when node 0 is changed. The rows of the next pair are read
  * while the current pair is scored, alternating between two row buffers (at
  * the end one pair of rows is read and not used). */
-template <int NS, int CS, int W, bool WRITE>
+template <int NS, int CS, int W, bool WRITE, bool T0 = false>
 __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
                                               const LogProbs &L, WaveLds &S, TableView tab,
                                               float (&R)[2], bool (&nan0)[2]) {
-  constexpr int SL = Slots<CS>::n;
+  static_assert(!T0 || CS == 2, "a half table: two full slots");
+  constexpr int SL = T0 ? 1 : Slots<CS>::n; /* the slots this pass scores */
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
   float *trow[2] = {nullptr, nullptr};
   if (WRITE) {
     trow[0] = tab.row(slot_cell<CS>(0));
-    trow[1] = tab.row(slot_cell<CS>(1));
+    if (!T0) trow[1] = tab.row(slot_cell<CS>(1));
   }
   uint64_t m[NS];
 #pragma unroll
@@ -892,15 +896,17 @@ __device__ __forceinline__ void changed_nodes(const uint64_t (&changed)[NS], con
  * where every node scores -0.0) into +0.0, which the ordered cell sum does
  * not see. The next group's reads are issued before the current group's
  * maxima; the empty asm with a memory clobber keeps them there. */
-template <int NS, int CS>
+template <int NS, int CS, bool T0 = false>
 __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_nodes,
                                                 TableView tab, const WaveLds &S,
                                                 float (&U)[2], bool (&nan0)[2]) {
-  constexpr int SL = Slots<CS>::n;
+  static_assert(!T0 || CS == 2, "a half table: two full slots");
+  constexpr int SL = T0 ? 1 : Slots<CS>::n; /* the slots the table holds */
   const uint32_t lane = lane_id();
   const uint32_t hi_half = lane >> 5;
   const uint32_t gh = (n_nodes + 7u) >> 3; /* 4-node groups per half */
-  const float *const rj[2] = {tab.row(slot_cell<CS>(0)), tab.row(slot_cell<CS>(1))};
+  const float *const rj[2] = {tab.row(slot_cell<CS>(0)),
+                              T0 ? tab.row(slot_cell<CS>(0)) : tab.row(slot_cell<CS>(1))};
   const float *const rh = rj[1] + (hi_half ? 4u * gh : 0u);
   const float *const qh = S.pen + (hi_half ? 4u * gh : 0u);
   if (!node0_changed) {
@@ -997,7 +1003,7 @@ __device__ __forceinline__ void unchanged_nodes(bool node0_changed, uint32_t n_n
  * not depend on the order the nodes are visited in (the first-wins rule
  * only matters for a NaN at node 0, kept by the flags), so the result is
  * bit-identical to the full scorer. */
-template <int NS, int CS, int W, bool SPL = false>
+template <int NS, int CS, int W, bool SPL = false, bool T0 = false>
 __device__ __forceinline__ float score_changed(const uint64_t (&changed)[NS], const Cells<CS, W> &c,
                                                uint32_t n_nodes, uint32_t n_cells,
                                                const LogProbs &L, WaveLds &S, TableView tab,
@@ -1005,9 +1011,19 @@ __device__ __forceinline__ float score_changed(const uint64_t (&changed)[NS], co
   const float ninf = __int_as_float(0xff800000);
   float R[2] = {ninf, ninf}, U[2] = {ninf, ninf};
   bool nan0[2] = {false, false};
-  changed_nodes<NS, CS, W, false>(changed, c, L, S, tab, R, nan0);
-  unchanged_nodes<NS, CS>((changed[0] & 1ull) != 0ull, n_nodes, tab, S, U, nan0);
+  changed_nodes<NS, CS, W, false, T0>(changed, c, L, S, tab, R, nan0);
+  unchanged_nodes<NS, CS, T0>((changed[0] & 1ull) != 0ull, n_nodes, tab, S, U, nan0);
   float best[2] = {vmax(R[0], U[0]), vmax(R[1], U[1])};
+  if constexpr (T0) {
+    /* the table holds slot 0 only: slot 1 over every node of the proposal
+     * (its rows are in S.rows) */
+    float b1[2];
+    bool n1[2];
+    uint32_t u01[2], u11[2];
+    node_maxima<NS, CS, W, false, false, 1>(c, n_nodes, L, S, tab, b1, n1, u01, u11);
+    best[1] = b1[1];
+    nan0[1] = n1[1];
+  }
   if constexpr (SPL) {
     static_assert(CS == 1, "a split chain holds one cell slot per wave");
     return finish_sum_split(best[0], nan0[0], n_cells, sp);

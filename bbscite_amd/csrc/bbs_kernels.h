@@ -91,8 +91,9 @@ struct BatchArgs {
 void shape_of(uint32_t n_genes, uint32_t n_cells, int *ns, int *cs, int *w,
               bool paired);
 /* table_bytes: 0 for the full scorer every step, else the dynamic LDS of an
- * incremental chain wave (chain_table_bytes). */
-hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
+ * incremental chain wave (chain_table_bytes, or chain_half_table_bytes with
+ * half: a table of the first 64 cells, the others rescored every step). */
+hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes, bool half,
                         hipStream_t stream);
 /* A chain split over `split` (2 or 4) waves by cells, for launches with
  * fewer chains than SIMDs: LDS bytes of its workgroup, 0 if the split does
@@ -110,6 +111,9 @@ constexpr uint32_t kMaxGroupLds = 163840u;
 /* LDS bytes of an incremental chain wave (pair-score table, tree rows,
  * sums, node penalties), 0 if it exceeds the LDS of one workgroup. */
 uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells);
+/* LDS bytes of a chain wave with the table of cells 0..63 only, for
+ * datasets of over 96 cells (two full cell slots); 0 otherwise. */
+uint32_t chain_half_table_bytes(uint32_t n_nodes, uint32_t n_cells);
 /* Static LDS of an incremental chain wave besides chain_table_bytes. */
 uint32_t chain_static_lds();
 /* cells32: [128][10] words for the lane-per-tree kernel (used when counts

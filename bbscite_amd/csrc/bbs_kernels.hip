@@ -341,11 +341,14 @@ struct ChainView {
  * count pairs on the matrix cores where the dataset fills its tiles (the
  * plain kernel; the batched kernel, mostly small datasets, keeps its
  * registers for the vector path). */
-template <int NS, int CS, int W, bool TRACE, bool INCR, bool MFMA, bool SPL = false>
+template <int NS, int CS, int W, bool TRACE, bool INCR, bool MFMA, bool SPL = false,
+          bool HALF = false>
 __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, TableView tab,
                                           const CellSplit &split = CellSplit{nullptr, 0u, 0u},
                                           bool leader = true) {
   static_assert(!SPL || (CS == 1 && INCR && !MFMA), "a split chain: one cell slot, table");
+  static_assert(!HALF || (CS == 2 && INCR && !MFMA && !SPL),
+                "a half table: two full cell slots, the table of slot 0");
   const uint32_t lane = lane_id();
   const uint32_t N = a.n_nodes;
   /* SPL: this wave scores the cells a.cells[0, a.n_cells) of a chain split
@@ -396,9 +399,9 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       store_rows<NS>(dm, N, lds);
       t = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, L, lds, tab);
     } else {
-      t = score_tree<NS, CS, W, false, INCR, SPL>(dm, cells, N, a.n_cells, L, lds, unused01,
-                                                  unused11, INCR ? tab : TableView{nullptr, 0u, 0u},
-                                                  SPL ? next_split() : split);
+      t = score_tree<NS, CS, W, false, INCR, SPL, HALF>(
+          dm, cells, N, a.n_cells, L, lds, unused01, unused11,
+          INCR ? tab : TableView{nullptr, 0u, 0u}, SPL ? next_split() : split);
     }
     cur_score = a.score_init ? t + cur_prior : *a.score;
   } else {
@@ -460,8 +463,8 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       }
       store_pen<NS>(diff, N, lds);
       store_rows<NS>(pdm, N, lds);
-      prop_raw = score_changed<NS, CS, W, SPL>(changed, cells, N, a.n_cells, Lp, lds, tab,
-                                               SPL ? next_split() : split);
+      prop_raw = score_changed<NS, CS, W, SPL, HALF>(changed, cells, N, a.n_cells, Lp, lds, tab,
+                                                     SPL ? next_split() : split);
     } else {
       /* beta moves (or full scoring): with INCR the proposal's pair scores
        * go straight into the table, since most beta moves are accepted; a
@@ -470,10 +473,9 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
         store_rows<NS>(pdm, N, lds);
         prop_raw = mfma_maxima<NS, CS, W, true>(mc, N, a.n_cells, Lp, lds, tab);
       } else {
-        prop_raw = score_tree<NS, CS, W, false, INCR, SPL>(pdm, cells, N, a.n_cells, Lp, lds,
-                                                           unused01, unused11,
-                                                           INCR ? tab : TableView{nullptr, 0u, 0u},
-                                                           SPL ? next_split() : split);
+        prop_raw = score_tree<NS, CS, W, false, INCR, SPL, HALF>(
+            pdm, cells, N, a.n_cells, Lp, lds, unused01, unused11,
+            INCR ? tab : TableView{nullptr, 0u, 0u}, SPL ? next_split() : split);
       }
     }
     const float prop_score = prop_raw + prop_prior;
@@ -501,7 +503,8 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
       } else {
         float best[2];
         bool n0[2];
-        node_maxima<NS, CS, W, false, true>(cells, N, L, lds, tab, best, n0, unused01, unused11);
+        node_maxima<NS, CS, W, false, true, 0, HALF>(cells, N, L, lds, tab, best, n0, unused01,
+                                                     unused11);
       }
       wave_lds_sync();
     }
@@ -510,7 +513,7 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
         /* the table follows the accepted tree (its rows are in lds.rows) */
         float R[2];
         bool n0[2];
-        changed_nodes<NS, CS, W, true>(changed, cells, Lp, lds, tab, R, n0);
+        changed_nodes<NS, CS, W, true, HALF>(changed, cells, Lp, lds, tab, R, n0);
         wave_lds_sync();
       }
       cur_score = prop_score;
@@ -551,7 +554,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
  * at least scan_reach floats. */
 struct ChainLds {
   uint32_t table, rows, sums, pen, total;
-  __host__ __device__ static ChainLds of(uint32_t n_nodes, uint32_t n_cells, int cs) {
+  /* half: a table of the first 64 cells only (two full cell slots, the
+   * second rescored every step) */
+  __host__ __device__ static ChainLds of(uint32_t n_nodes, uint32_t n_cells, int cs,
+                                         bool half = false) {
     const uint32_t stride = TableView::stride_for(n_nodes);
     const uint32_t slots = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
     const uint32_t pen_floats = TableView::pen_len(n_nodes) > TableView::scan_reach(n_nodes)
@@ -559,7 +565,7 @@ struct ChainLds {
                                     : TableView::scan_reach(n_nodes);
     ChainLds l;
     l.table = 0u;
-    l.rows = TableView::floats_for(n_nodes, n_cells, slots);
+    l.rows = half ? 64u * stride : TableView::floats_for(n_nodes, n_cells, slots);
     l.sums = l.rows + 4u * n_nodes;
     l.pen = l.sums + slots;
     l.total = l.pen + pen_floats;
@@ -567,7 +573,7 @@ struct ChainLds {
   }
 };
 
-template <int NS, int CS, int W, bool TRACE, bool INCR>
+template <int NS, int CS, int W, bool TRACE, bool INCR, bool HALF = false>
 __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   extern __shared__ __align__(16) float dyn[]; /* ChainLds with INCR */
   const uint32_t chain = blockIdx.x;
@@ -602,7 +608,13 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.gamma = a.gamma;
   v.prior = a.prior;
   v.probs = a.probs;
-  if constexpr (INCR) {
+  if constexpr (INCR && HALF) {
+    /* the table of cells 0..63 (slot 0); slot 1 is rescored every step */
+    const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS, true);
+    WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
+    run_chain<NS, CS, W, TRACE, true, false, false, true>(
+        v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes), 64u});
+  } else if constexpr (INCR) {
     const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
     run_chain<NS, CS, W, TRACE, true, true>(
@@ -1559,12 +1571,32 @@ uint32_t chain_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
   return bytes <= kMaxGroupLds ? bytes : 0u;
 }
 
+uint32_t chain_half_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
+  int ns, cs, w;
+  shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
+  if (cs != 2) return 0u;
+  const uint32_t bytes = 4u * ChainLds::of(n_nodes, n_cells, cs, true).total;
+  return bytes <= kMaxGroupLds ? bytes : 0u;
+}
+
 uint32_t chain_static_lds() { return 0u; }
 
 template <int NS, int CS, int W>
 static hipError_t launch_chain_shape(const ChainArgs &a, bool trace, uint32_t table_bytes,
-                                     hipStream_t stream) {
+                                     bool half, hipStream_t stream) {
   const dim3 grid(a.n_chains), block(64);
+  if (half) {
+    if constexpr (CS == 2) {
+      if (trace)
+        hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, true, true>), grid, block, table_bytes,
+                           stream, a);
+      else
+        hipLaunchKernelGGL((chain_kernel<NS, CS, W, false, true, true>), grid, block,
+                           table_bytes, stream, a);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   if (table_bytes) {
     if (trace)
       hipLaunchKernelGGL((chain_kernel<NS, CS, W, true, true>), grid, block, table_bytes, stream,
@@ -1581,15 +1613,16 @@ static hipError_t launch_chain_shape(const ChainArgs &a, bool trace, uint32_t ta
   return hipGetLastError();
 }
 
-hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes,
+hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes, bool half,
                         hipStream_t stream) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
-  if (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, a.n_cells, cs).total)
+  if ((half && (cs != 2 || table_bytes == 0u)) ||
+      (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, a.n_cells, cs, half).total))
     return hipErrorInvalidValue;
 #define X(NS_, CS_, W_)                                                        \
   if (ns == NS_ && cs == CS_ && w == W_)                                       \
-    return launch_chain_shape<NS_, CS_, W_>(a, trace, table_bytes, stream);
+    return launch_chain_shape<NS_, CS_, W_>(a, trace, table_bytes, half, stream);
   BBS_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

@@ -529,22 +529,54 @@ int bbs_begin(bbs_ctx *ctx) {
  * full rescoring. Automatic mode takes the table when it leaves every CU
  * room for as many chain waves as the launch can give it (at most one per
  * SIMD), so the table never costs concurrency. */
-static uint32_t chain_table(const bbs_ctx *ctx) {
-  if (ctx->scoring_mode == BBS_SCORING_FULL) return 0u;
+/* The pair-score table of this context's plain launches: its LDS bytes per
+ * chain wave (0: every step rescores every pair) and whether it holds the
+ * first 64 cells only (half; the second cell slot is then rescored every
+ * step). AUTO keeps the whole table when it costs no concurrent chains (as
+ * many chains run at once with it as without, one per SIMD), else the half
+ * table on the same terms (over 96 cells) when every chain fits one per
+ * SIMD, else none. Beyond that, full rescoring runs two waves per SIMD
+ * without LDS: at 127 x 128 with 2,048 chains it took 228.5 ms per 10,000
+ * steps against 383.8 ms with the half table. */
+struct TableChoice {
+  uint32_t bytes;
+  bool half;
+};
+
+static TableChoice chain_table_choice(const bbs_ctx *ctx) {
+  if (ctx->scoring_mode == BBS_SCORING_FULL) return {0u, false};
+  if (ctx->scoring_mode == BBS_SCORING_HALF_TABLE)
+    return {bbs::chain_half_table_bytes(ctx->n_nodes, ctx->n_cells), true};
   const uint32_t t = bbs::chain_table_bytes(ctx->n_nodes, ctx->n_cells);
-  if (t == 0u || ctx->scoring_mode == BBS_SCORING_INCREMENTAL) return t;
-  const uint32_t per_wave = t + bbs::chain_static_lds();
+  if (t == 0u || ctx->scoring_mode == BBS_SCORING_INCREMENTAL) return {t, false};
   const uint32_t cus = ctx->n_cus ? ctx->n_cus : 1u;
   uint32_t want = (ctx->params.n_chains + cus - 1u) / cus;
   if (want > 4u) want = 4u;
   if (want == 0u) want = 1u;
-  return (uint64_t)per_wave * want <= ctx->lds_per_cu ? t : 0u;
+  auto fits = [&](uint32_t bytes) {
+    return (uint64_t)(bytes + bbs::chain_static_lds()) * want <= ctx->lds_per_cu;
+  };
+  if (fits(t)) return {t, false};
+  const uint32_t h = bbs::chain_half_table_bytes(ctx->n_nodes, ctx->n_cells);
+  if (h != 0u && ctx->params.n_chains <= 4u * cus && fits(h)) return {h, true};
+  return {0u, false};
+}
+
+/* The whole table's bytes if this context's launches keep one, else 0. */
+static uint32_t chain_table(const bbs_ctx *ctx) {
+  const TableChoice c = chain_table_choice(ctx);
+  return c.half ? 0u : c.bytes;
 }
 
 int bbs_set_scoring_mode(bbs_ctx *ctx, int mode) {
   if (!ctx) return fail(nullptr, BBS_ERR_ARG, "ctx is NULL");
-  if (mode != BBS_SCORING_AUTO && mode != BBS_SCORING_FULL && mode != BBS_SCORING_INCREMENTAL)
+  if (mode != BBS_SCORING_AUTO && mode != BBS_SCORING_FULL && mode != BBS_SCORING_INCREMENTAL &&
+      mode != BBS_SCORING_HALF_TABLE)
     return fail(ctx, BBS_ERR_ARG, "unknown scoring mode %d", mode);
+  if (mode == BBS_SCORING_HALF_TABLE &&
+      bbs::chain_half_table_bytes(ctx->n_nodes, ctx->n_cells) == 0u)
+    return fail(ctx, BBS_ERR_SIZE,
+                "no half table for %u cells (it serves datasets of over 96 cells)", ctx->n_cells);
   if (mode == BBS_SCORING_INCREMENTAL && bbs::chain_table_bytes(ctx->n_nodes, ctx->n_cells) == 0u)
     return fail(ctx, BBS_ERR_SIZE,
                 "no incremental scorer for %u genes x %u cells (table over the LDS of a workgroup)",
@@ -555,7 +587,9 @@ int bbs_set_scoring_mode(bbs_ctx *ctx, int mode) {
 
 /* Waves per chain of this context's launches (bbs_set_chain_split). */
 static uint32_t chain_split(const bbs_ctx *ctx) {
-  if (ctx->scoring_mode == BBS_SCORING_FULL) return 1u; /* the split keeps a table */
+  /* the split keeps a whole table */
+  if (ctx->scoring_mode == BBS_SCORING_FULL || ctx->scoring_mode == BBS_SCORING_HALF_TABLE)
+    return 1u;
   if (ctx->chain_split != 0u) return ctx->chain_split;
   if (chain_table(ctx) == 0u) return 1u;
   /* Splitting the cells shortens a wave's step only where one wave would
@@ -577,7 +611,9 @@ static uint32_t chain_split(const bbs_ctx *ctx) {
  * chain: 63 x 64 +21 %, 94 x 95 +51 % (8 waves), 127 x 128 +25 % (4 waves;
  * the cell split gives +20 % there); 2 waves gain little. */
 static uint32_t chain_speculation(const bbs_ctx *ctx) {
-  if (ctx->scoring_mode == BBS_SCORING_FULL) return 1u; /* speculation keeps a table */
+  /* speculation keeps a whole table */
+  if (ctx->scoring_mode == BBS_SCORING_FULL || ctx->scoring_mode == BBS_SCORING_HALF_TABLE)
+    return 1u;
   if (ctx->speculation != 0u) return ctx->speculation;
   if (chain_table(ctx) == 0u) return 1u;
   const uint64_t simds = 4ull * (ctx->n_cus ? ctx->n_cus : 1u);
@@ -658,7 +694,7 @@ int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   a.gamma = ctx->params.gamma;
   a.prior = ctx->prior;
   a.probs = ctx->probs;
-  const uint32_t table = chain_table(ctx);
+  const TableChoice table = chain_table_choice(ctx);
   const uint32_t spec = chain_speculation(ctx);
   const uint32_t split = spec > 1u ? 1u : chain_split(ctx);
   const bool traced = ctx->n_traced > 0u || ctx->n_prop_traced > 0u;
@@ -668,7 +704,7 @@ int bbs_advance(bbs_ctx *ctx, uint32_t n, float *makespan_ms) {
   else if (a.n_chains > 0u && split > 1u)
     HIP_TRY(ctx, bbs::launch_chain_split(a, traced, split, ctx->stream));
   else if (a.n_chains > 0u)
-    HIP_TRY(ctx, bbs::launch_chain(a, traced, table, ctx->stream));
+    HIP_TRY(ctx, bbs::launch_chain(a, traced, table.bytes, table.half, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   float ms = 0.f;

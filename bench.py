@@ -542,6 +542,31 @@ def full_device_split_bench(steps=20_000):
     return out
 
 
+def scoring_modes_bench(steps=10_000):
+    """Plain launches over 96 cells, 1,024 and 2,048 chains: full rescoring,
+    the half table (cells 0..63), the whole table and the automatic choice,
+    per launch of `steps` steps (the second of two runs)."""
+    out = []
+    for G, C, chains in ((127, 128, 1024), (100, 110, 1024), (120, 100, 1024), (127, 128, 2048)):
+        m, _, _ = B.generate_dataset(7, G, C, 1e-4, 0.3, 0.15)
+        mut, kno = B.encode_matrix(m)
+        p = B.default_params(n_chains=chains, n_steps=steps, seed=7, alpha_mean=1e-4,
+                             beta_mean=0.3, beta_sd=0.05, prob_beta_change=0.1)
+        row = {"genes": G, "cells": C, "chains": chains, "steps": steps}
+        best = set()
+        for mode in ("full", "half", "incremental", "auto"):
+            with B.Context(p, mut, kno, G) as ctx:
+                ctx.set_scoring_mode(mode)
+                ctx.run()
+                row[f"{mode}_ms"] = round(ctx.run(), 2)
+                best.add(np.float32(ctx.best()[3]).tobytes())
+        row["best_ll_identical"] = len(best) == 1
+        out.append(row)
+        log(f"[bench] {G}x{C} {chains} chains: " + ", ".join(
+            f"{k} {row[k + '_ms']:.1f} ms" for k in ("full", "half", "incremental", "auto")))
+    return out
+
+
 def layout_code(genes, cells):
     """Register layout (node slots, cell slots, gene words) as the library's
     shape_of picks it, as the code NS*100 + CS*10 + W."""
@@ -666,6 +691,8 @@ def main():
     ap.add_argument("--serve-cap", type=int, default=20_000)
     ap.add_argument("--round-ms", type=float, default=None,
                     help="server slice length (default: the library's)")
+    ap.add_argument("--scoring-modes", action="store_true",
+                    help="only time the scoring modes over 96 cells (one JSON line)")
     ap.add_argument("--split-full-device", action="store_true",
                     help="only time waves per chain with every SIMD busy (one JSON line)")
     ap.add_argument("--valu-peak-only", action="store_true",
@@ -683,6 +710,9 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         log(f"[bench] copied {a.copy_calibration} MiB")
+        return
+    if a.scoring_modes:
+        print(json.dumps({"scoring_modes": scoring_modes_bench()}), flush=True)
         return
     if a.split_full_device:
         print(json.dumps({"split_full_device": full_device_split_bench()}), flush=True)

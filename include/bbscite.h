@@ -125,6 +125,11 @@ int bbs_set_proposal_trace(bbs_ctx *ctx, uint32_t n_traced_chains);
 #define BBS_SCORING_AUTO 0
 #define BBS_SCORING_FULL 1
 #define BBS_SCORING_INCREMENTAL 2
+/* HALF_TABLE (datasets of over 96 cells): the table holds cells 0..63 and
+ * the other cells are rescored every step, in about half the LDS. AUTO
+ * takes it where the whole table would cost concurrent chains and the half
+ * table does not. BBS_ERR_SIZE for 96 cells or fewer. */
+#define BBS_SCORING_HALF_TABLE 3
 int bbs_set_scoring_mode(bbs_ctx *ctx, int mode);
 
 /* Waves per chain of the plain launch (no reference counterpart: a launch

@@ -164,19 +164,25 @@ CHAIN_CASES = [
     # half the moves change beta: rounds of the speculative chain with
     # several beta proposals, accepted and rejected ones
     (20, 70, 12, 3, 300, 0.5, 0.3, 0.1),
+    # over 96 cells (two full cell slots; the half table), one and two node
+    # slots, many beta moves
+    (100, 110, 13, 3, 200, 0.1, 0.3, 0.1),
+    (50, 97, 14, 2, 250, 0.4, 0.25, 0.1),
+    (10, 120, 15, 2, 300, 0.2, 0.3, 0.1),
 ]
 
 
-MODES = ["full", "incremental", "split2", "split4", "spec2", "spec4", "spec8"]
+MODES = ["full", "incremental", "half", "split2", "split4", "spec2", "spec4", "spec8"]
 
 
 def _set_mode(ctx, mode):
     """full: every pair rescored; incremental: the pair-score table, one
-    wave per chain; split2 / split4: the table with each chain split over 2
-    or 4 waves by cells; spec2 / spec4 / spec8: the table with 2, 4 or 8
-    waves per chain proposing consecutive steps at once."""
+    wave per chain; half: the table of cells 0..63 only, the others rescored
+    every step (over 96 cells); split2 / split4: the table with each chain
+    split over 2 or 4 waves by cells; spec2 / spec4 / spec8: the table with
+    2, 4 or 8 waves per chain proposing consecutive steps at once."""
     try:
-        ctx.set_scoring_mode("full" if mode == "full" else "incremental")
+        ctx.set_scoring_mode(mode if mode in ("full", "half") else "incremental")
         if mode != "full":
             ctx.set_speculation(int(mode[4:]) if mode.startswith("spec") else 1)
             ctx.set_chain_split(int(mode[5:]) if mode.startswith("split") else 1)
@@ -246,7 +252,7 @@ def test_chain_trace_replay(case, mode):
     assert {1, 2, 3} <= types and (pb == 0 or 0 in types)
 
 
-@pytest.mark.parametrize("G,C", [(94, 95), (63, 64), (127, 96), (30, 128), (5, 7)])
+@pytest.mark.parametrize("G,C", [(94, 95), (63, 64), (127, 96), (30, 128), (5, 7), (120, 110)])
 def test_incremental_equals_full_over_long_runs(G, C):
     """Thousands of steps with many accepted moves of every kind: the
     pair-score table stays exact, so both scoring modes give identical
@@ -394,6 +400,10 @@ def test_errors_are_reported():
         with pytest.raises(B.BbsError) as e:
             ctx.proposal_trace(0)
         assert e.value.code == B.ERR_STATE
+        # the half table serves datasets of over 96 cells only
+        with pytest.raises(B.BbsError) as e:
+            ctx.set_scoring_mode("half")
+        assert e.value.code == B.ERR_SIZE
         ctx.begin()
         with pytest.raises(B.BbsError):
             ctx.advance(11)
