@@ -521,8 +521,10 @@ __device__ __forceinline__ void node_maxima(const Cells<CS, W> &c, uint32_t n_no
     trow[0] = tab.row(slot_cell<CS>(0));
     if (!T0) trow[1] = tab.row(slot_cell<CS>(1));
   }
-#pragma unroll
-  for (int j = J0; j < 2; j++) nan0[j] = false;
+  if constexpr (J0 == 0)
+    nan0[0] = nan0[1] = false;
+  else
+    nan0[1] = false;
   /* one node for the full slots (from slot J0 on) */
   auto full = [&](const uint4 &A, uint32_t node, bool first) {
 #pragma unroll

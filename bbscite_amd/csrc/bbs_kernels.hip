@@ -554,10 +554,10 @@ __device__ __forceinline__ void run_chain(const ChainView &a, WaveLds &lds, Tabl
  * at least scan_reach floats. */
 struct ChainLds {
   uint32_t table, rows, sums, pen, total;
-  /* half: a table of the first 64 cells only (two full cell slots, the
+  /* HALF: a table of the first 64 cells only (two full cell slots, the
    * second rescored every step) */
-  __host__ __device__ static ChainLds of(uint32_t n_nodes, uint32_t n_cells, int cs,
-                                         bool half = false) {
+  template <bool HALF = false>
+  __host__ __device__ static ChainLds of(uint32_t n_nodes, uint32_t n_cells, int cs) {
     const uint32_t stride = TableView::stride_for(n_nodes);
     const uint32_t slots = cs == 1 ? 64u : cs == 3 ? 96u : 128u;
     const uint32_t pen_floats = TableView::pen_len(n_nodes) > TableView::scan_reach(n_nodes)
@@ -565,7 +565,10 @@ struct ChainLds {
                                     : TableView::scan_reach(n_nodes);
     ChainLds l;
     l.table = 0u;
-    l.rows = half ? 64u * stride : TableView::floats_for(n_nodes, n_cells, slots);
+    if constexpr (HALF)
+      l.rows = 64u * stride;
+    else
+      l.rows = TableView::floats_for(n_nodes, n_cells, slots);
     l.sums = l.rows + 4u * n_nodes;
     l.pen = l.sums + slots;
     l.total = l.pen + pen_floats;
@@ -610,7 +613,7 @@ __global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   v.probs = a.probs;
   if constexpr (INCR && HALF) {
     /* the table of cells 0..63 (slot 0); slot 1 is rescored every step */
-    const ChainLds l = ChainLds::of(a.n_nodes, a.n_cells, CS, true);
+    const ChainLds l = ChainLds::of<true>(a.n_nodes, a.n_cells, CS);
     WaveLds lds{reinterpret_cast<uint4 *>(dyn + l.rows), dyn + l.sums, dyn + l.pen};
     run_chain<NS, CS, W, TRACE, true, false, false, true>(
         v, lds, TableView{dyn + l.table, TableView::stride_for(a.n_nodes), 64u});
@@ -1575,7 +1578,7 @@ uint32_t chain_half_table_bytes(uint32_t n_nodes, uint32_t n_cells) {
   int ns, cs, w;
   shape_of(n_nodes - 1u, n_cells, &ns, &cs, &w, true);
   if (cs != 2) return 0u;
-  const uint32_t bytes = 4u * ChainLds::of(n_nodes, n_cells, cs, true).total;
+  const uint32_t bytes = 4u * ChainLds::of<true>(n_nodes, n_cells, cs).total;
   return bytes <= kMaxGroupLds ? bytes : 0u;
 }
 
@@ -1617,8 +1620,9 @@ hipError_t launch_chain(const ChainArgs &a, bool trace, uint32_t table_bytes, bo
                         hipStream_t stream) {
   int ns, cs, w;
   shape_of(a.n_nodes - 1u, a.n_cells, &ns, &cs, &w, true);
-  if ((half && (cs != 2 || table_bytes == 0u)) ||
-      (table_bytes && table_bytes < 4u * ChainLds::of(a.n_nodes, a.n_cells, cs, half).total))
+  const uint32_t need = half ? ChainLds::of<true>(a.n_nodes, a.n_cells, cs).total
+                             : ChainLds::of(a.n_nodes, a.n_cells, cs).total;
+  if ((half && (cs != 2 || table_bytes == 0u)) || (table_bytes && table_bytes < 4u * need))
     return hipErrorInvalidValue;
 #define X(NS_, CS_, W_)                                                        \
   if (ns == NS_ && cs == CS_ && w == W_)                                       \
