@@ -230,16 +230,16 @@ def cpu_scoring_baseline(ll_gpu, seconds_budget=5.0):
     }
 
 
-def config4_jobs(steps, chains=64, n_datasets=256):
+def batch_mix_jobs(steps, chains=64, n_datasets=256):
     return WL.batch_mix(steps, chains, n_datasets)
 
 
 def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=20_000):
-    """Config 4: the 256-dataset uneven mix, plain launches vs the batched
+    """The 256-dataset uneven mix, plain launches vs the batched
     per-layout launches. Plain launches run one dataset each, serially. At
     the plain step count the batch is also timed as an eager run and as a
     replay of its hipGraph capture."""
-    jobs_plain = config4_jobs(steps_plain)
+    jobs_plain = batch_mix_jobs(steps_plain)
     t_plain = 0.0
     bests_plain = []
     for p, mut, kno, G in jobs_plain:
@@ -247,7 +247,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=20_000):
             t_plain += ctx.run() / 1e3
             bests_plain.append(ctx.best()[3])
     plain_steps = sum(p.n_chains * p.n_steps for p, *_ in jobs_plain)
-    cfg, probe_ms = B.batch_autotune(config4_jobs(probe_steps), probe_steps)
+    cfg, probe_ms = B.batch_autotune(batch_mix_jobs(probe_steps), probe_steps)
     log(f"[bench] batch autotune: wpg={cfg.waves_per_group} streams={cfg.n_streams} "
         f"order={cfg.order} two-wave layouts {two_wave_codes(cfg.two_wave_layouts)} "
         f"({probe_ms:.1f} ms at {probe_steps} steps)")
@@ -264,7 +264,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=20_000):
         graph_short = min(g.run() for _ in range(3))
         same_graph = all(np.float32(g.best(j)[3]).tobytes() == np.float32(bests_plain[j]).tobytes()
                          for j in range(len(jobs_plain)))
-    jobs = config4_jobs(steps_batched)
+    jobs = batch_mix_jobs(steps_batched)
     # datasets whose timed batched result is checked against a plain run of
     # the same step count: the smallest, the largest and two in between
     by_size = sorted(range(len(jobs)), key=lambda j: (jobs[j][3] + 1) * jobs[j][1].shape[0])
@@ -316,7 +316,7 @@ def batched_bench(steps_batched=200_000, steps_plain=2_000, probe_steps=20_000):
     }
 
 
-def config5_jobs(n_jobs, seed=5):
+def served_stream_jobs(n_jobs, seed=5):
     return WL.served_stream(n_jobs, seed)
 
 
@@ -328,7 +328,7 @@ def serving_bench(n_jobs, latencies_path=None, deadline_s=None, admission=None, 
     time: each job is submitted at its arrival time; latency = completion -
     arrival. With deadline_s the run stops after that many seconds and
     reports the jobs completed by then (complete: false)."""
-    jobs = config5_jobs(n_jobs)
+    jobs = served_stream_jobs(n_jobs)
     log(f"[bench] serving {n_jobs} jobs")
     lat = []
     arrival = {}
@@ -725,7 +725,7 @@ def main():
         print(json.dumps({"int32_ops_per_s": ops, "valu_wave_insts_per_s": insts}), flush=True)
         return
     if a.batch_shapes:
-        jobs = config4_jobs(a.batch_steps)
+        jobs = batch_mix_jobs(a.batch_steps)
         res = []
         for shape in a.batch_shapes.split(","):
             wpg, nst = (int(x) for x in shape.split(":"))
