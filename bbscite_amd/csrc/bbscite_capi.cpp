@@ -606,21 +606,28 @@ static uint32_t chain_split(const bbs_ctx *ctx) {
 }
 
 /* Speculative waves per chain of this context's launches (bbs_set_speculation).
- * Automatically 8 or 4 when every chain's waves fit the device's SIMDs at
- * once and the table is in use. Measured with 8 chains against one wave per
- * chain: 63 x 64 +21 %, 94 x 95 +51 % (8 waves), 127 x 128 +25 % (4 waves;
- * the cell split gives +20 % there); 2 waves gain little. */
+ * Automatically the largest k of 8, 4 and 2 whose waves run at most two per
+ * SIMD, with every chain's LDS at once, while the table is in use. Measured
+ * per launch of 10,000-20,000 steps with 10 % beta moves, against one wave
+ * per chain: 63 x 64 with 128 / 256 / 512 / 1,024 chains 28 / 29 / 25 / 4 %
+ * faster (k = 8 / 8 / 4 / 2), 94 x 95 with 128 / 256 chains 28 / 29 %;
+ * where two chains' slabs do not fit a CU (94 x 95 and 127 x 128 from 512
+ * chains) every k was slower, since fewer chains then run at once. */
 static uint32_t chain_speculation(const bbs_ctx *ctx) {
   /* speculation keeps a whole table */
   if (ctx->scoring_mode == BBS_SCORING_FULL || ctx->scoring_mode == BBS_SCORING_HALF_TABLE)
     return 1u;
   if (ctx->speculation != 0u) return ctx->speculation;
   if (chain_table(ctx) == 0u) return 1u;
-  const uint64_t simds = 4ull * (ctx->n_cus ? ctx->n_cus : 1u);
-  for (uint32_t k = 8u; k >= 4u; k /= 2u)
-    if ((uint64_t)ctx->params.n_chains * k <= simds &&
-        bbs::chain_spec_bytes(ctx->n_nodes, ctx->n_cells, k) != 0u)
+  const uint64_t cus = ctx->n_cus ? ctx->n_cus : 1u;
+  const uint64_t chains = ctx->params.n_chains;
+  const uint64_t per_cu = (chains + cus - 1u) / cus;
+  for (uint32_t k = 8u; k >= 2u; k /= 2u) {
+    const uint64_t bytes = bbs::chain_spec_bytes(ctx->n_nodes, ctx->n_cells, k);
+    if (bytes != 0u && chains * k <= 8u * cus && per_cu * k <= 8u &&
+        per_cu * bytes <= ctx->lds_per_cu)
       return k;
+  }
   return 1u;
 }
 

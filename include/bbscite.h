@@ -159,9 +159,10 @@ int bbs_set_chain_split(bbs_ctx *ctx, uint32_t waves_per_chain);
  * first accepted one are kept and the rest proposed again from the new tree,
  * so a round advances the chain by up to k steps in about the time of one.
  * A round lasts as long as its slowest proposal (a beta move rescores every
- * pair), so the gain is below k. 1: off. 0 (the default): 8 or 4 when every
- * chain's waves fit the device's SIMDs at once and the incremental scorer is
- * in use; when it applies it takes precedence over bbs_set_chain_split.
+ * pair), so the gain is below k. 1: off. 0 (the default): the largest of 8,
+ * 4 and 2 whose waves run at most two per SIMD with every chain's LDS at
+ * once, while the incremental scorer is in use, else 1; when it applies it
+ * takes precedence over bbs_set_chain_split.
  * Trajectories, traces and
  * bests are bit-identical for every value; with BBS_SCORING_FULL it is off.
  * BBS_ERR_ARG for other values, BBS_ERR_SIZE if the k waves' LDS does not fit
