@@ -252,6 +252,27 @@ def test_chain_trace_replay(case, mode):
     assert {1, 2, 3} <= types and (pb == 0 or 0 in types)
 
 
+@pytest.mark.parametrize("G,C,R", [(63, 64, 512), (63, 64, 1024), (94, 95, 256)])
+def test_automatic_speculation_with_many_chains(G, C, R):
+    """With hundreds of chains the default runs 8, 4 or 2 speculative waves
+    per chain (up to two waves per SIMD); every chain's results and final
+    tree equal one wave per chain."""
+    say(f"[auto spec] G={G} C={C} chains={R}")
+    p, mut, kno = _chain_case(G, C, 16, R, 300, 0.1, beta_mean=0.3, beta_sd=0.05)
+    out = []
+    for spec in (0, 1):
+        with B.Context(p, mut, kno, G) as ctx:
+            ctx.set_speculation(spec)
+            ctx.run()
+            out.append((ctx.chain_results(), ctx.chain_states(), ctx.best()))
+    (r0, s0, b0), (r1, s1, b1) = out
+    for x, y in zip(r0, r1):
+        assert np.array_equal(_bits(x) if x.dtype == np.float32 else x,
+                              _bits(y) if y.dtype == np.float32 else y)
+    assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
+    assert np.array_equal(b0[1], b1[1]) and _bits(b0[3]) == _bits(b1[3])
+
+
 @pytest.mark.parametrize("G,C", [(94, 95), (63, 64), (127, 96), (30, 128), (5, 7), (120, 110)])
 def test_incremental_equals_full_over_long_runs(G, C):
     """Thousands of steps with many accepted moves of every kind: the
