@@ -169,6 +169,9 @@ CHAIN_CASES = [
     (100, 110, 13, 3, 200, 0.1, 0.3, 0.1),
     (50, 97, 14, 2, 250, 0.4, 0.25, 0.1),
     (10, 120, 15, 2, 300, 0.2, 0.3, 0.1),
+    # the most genes with two cells; nine entries in ten missing
+    (127, 2, 17, 2, 150, 0.1, 0.3, 0.1),
+    (30, 40, 18, 3, 200, 0.2, 0.25, 0.1, 0.9),
 ]
 
 
@@ -196,9 +199,10 @@ def _set_mode(ctx, mode):
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("case", CHAIN_CASES, ids=lambda c: f"G{c[0]}C{c[1]}")
 def test_chain_trace_replay(case, mode):
-    G, C, seed, R, S, pb, bm, bsd = case
+    G, C, seed, R, S, pb, bm, bsd = case[:8]
+    missing = case[8] if len(case) > 8 else 0.1
     say(f"[chain] G={G} C={C} chains={R} steps={S} {mode}")
-    p, mut, kno = _chain_case(G, C, seed, R, S, pb, beta_mean=bm, beta_sd=bsd)
+    p, mut, kno = _chain_case(G, C, seed, R, S, pb, beta_mean=bm, beta_sd=bsd, missing=missing)
     with B.Context(p, mut, kno, G) as ctx:
         _set_mode(ctx, mode)
         ctx.set_trace(R)
